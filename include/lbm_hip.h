@@ -1,0 +1,124 @@
+/*
+ * lbm_hip.h -- C ABI of liblbmhip.so, the MI355X (gfx950) implementation of
+ * the reference's D3Q19 TRT AA-pattern kernels.
+ *
+ * Drop-in entry points.  Each replaces one kernel-table function of the
+ * reference and has the same signature and argument meaning; the KernelData
+ * it returns carries Kernel/GetNode/SetNode/BoundaryConditions{Get,Set}Pdf
+ * function pointers with the reference's semantics (src/Kernel.h:187-211).
+ * Errors behave as in the reference: "ERROR: ..." on stdout and exit(1),
+ * unless LbmHipSetErrorMode(LBM_HIP_ERRORS_RETURN) was called, in which case
+ * Init leaves *kernelData NULL and LbmHipLastError() holds the message.
+ *
+ * Kernel arguments (the words after "--", Parameters.KernelArgs):
+ *   -blk <n> | -blk-x/-blk-y/-blk-z <n>   node-numbering blocks (list only,
+ *                                         src/BenchKernelD3Q19ListAaPvCommon.c:236-301)
+ *   -pad auto|no|<m>+<o>(,<m>+<o>)*       SoA node padding (list only,
+ *                                         src/Padding.c:19-193)
+ *   -force on|off     apply the Luo body force after every step, as the
+ *                     reference's VERIFICATION build does (default on)
+ *   -lambda <v>       TRT magic parameter (default 1/12, the value the
+ *                     reference hard-codes, src/BenchKernelD3Q19ListAaPv.c:177)
+ *   -bs <n>           threads per block of the step kernels (64..1024, step 64)
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#include "lbm_kernel_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-in kernel-table functions ------------------------------------ */
+
+/* Replaces D3Q19ListAaPvInit_PushSoA (src/BenchKernelD3Q19ListAaPv.h:32,
+ * table entry "list-aa-pv-soa" src/KernelFunctions.h:58-62).  Builds the
+ * fluid-node list and its 18-entry push adjacency on the GPU, bit-identical
+ * to the reference's list build, and allocates the device lattice. */
+void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters * params);
+
+/* Replaces D3Q19ListAaPvDeinit_PushSoA (src/BenchKernelD3Q19ListAaPv.h:34). */
+void D3Q19ListAaPvDeinit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData);
+
+/* Replaces D3Q19AaInit_AaSoA (src/BenchKernelD3Q19Aa.h:33, table entry
+ * "aa-soa" src/KernelFunctions.h:171-175). */
+void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters * params);
+
+/* Replaces D3Q19AaDeinit_AaSoA (src/BenchKernelD3Q19Aa.h:36). */
+void D3Q19AaDeinit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData);
+
+/* Kernel table of this library, laid out like g_kernels[]
+ * (src/KernelFunctions.h:56-188): "list-aa-pv-soa" and "aa-soa". */
+extern KernelFunctions g_lbmHipKernels[];
+extern const int g_lbmHipKernelCount;
+
+/* ---- extensions (no counterpart in the reference) ----------------------- */
+
+enum { LBM_HIP_ERRORS_EXIT = 0, LBM_HIP_ERRORS_RETURN = 1 };
+/* Selects the error behaviour of every entry point (process-wide). */
+void LbmHipSetErrorMode(int mode);
+/* Message of the last error, "" if none.  Cleared by LbmHipClearError. */
+const char * LbmHipLastError(void);
+void LbmHipClearError(void);
+
+/* Per-call timing of the last Kernel() call, from HIP events recorded on
+ * the kernel's stream around every step launch. */
+typedef struct {
+	int    steps;          /* steps run (MaxIterations rounded up to even) */
+	double loopMs;         /* first launch start to last launch end */
+	double evenMsAvg;      /* average duration of one even-step launch */
+	double oddMsAvg;       /* average duration of one odd-step launch */
+	double evenMsMin, oddMsMin;
+	double uploadMs;       /* host shadow -> device before the loop (0 if clean) */
+	double downloadMs;     /* device -> host shadow after the loop */
+} LbmHipTimings;
+int LbmHipGetTimings(const KernelData * kd, LbmHipTimings * out);
+
+/* Enables/disables the per-launch events (on by default).  With them off
+ * only loopMs is measured. */
+int LbmHipSetLaunchEvents(KernelData * kd, int on);
+/* Enables/disables the device -> host shadow copy at the end of Kernel()
+ * (on by default, needed by GetNode).  With it off, GetNode reads the state
+ * before the last Kernel() call until LbmHipSyncShadow is called. */
+int LbmHipSetShadowDownload(KernelData * kd, int on);
+int LbmHipSyncShadow(KernelData * kd);
+
+/* Sizes of the list kernel: fluid nodes, padded node stride nCells,
+ * adjacency row stride on the device, and the number of runs of nodes
+ * with consecutive adjacency (src/BenchKernelD3Q19ListAaPvCommon.c:326-469,
+ * without thread breaks). */
+typedef struct {
+	int nFluid;
+	int nCells;
+	int adjStride;
+	int nRuns;
+	int dims[3];
+	int periodic[3];
+	double bytesPerFlupEven;
+	double bytesPerFlupOdd;
+} LbmHipListInfo;
+int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out);
+
+/* Copies the device list build to caller buffers, in the reference's
+ * layouts (any pointer may be NULL):
+ *   adj    nFluid*18 uint32, node-major (KernelDataList.AdjList)
+ *   grid   nX*nY*nZ uint32, node id per cell, 0xffffffff for obstacles
+ *   coords 3*nFluid uint32 */
+int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, uint32_t * coords);
+
+/* Node views of every non-solid cell of ld, in mask order (x fastest),
+ * 19 values each, through kd->GetNode / kd->SetNode. */
+int LbmHipDumpNodes(KernelData * kd, const LatticeDesc * ld, PdfT * out);
+int LbmHipLoadNodes(KernelData * kd, const LatticeDesc * ld, const PdfT * in);
+
+/* Name and compute-unit count of the device the library runs on. */
+int LbmHipDeviceInfo(char * name, int nameLen, int * computeUnits, double * memClockGHz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,421 @@
+// lbm_aa.hip -- aa-soa on MI355X: full-array AA pattern with one ghost
+// layer, behind the reference's kernel-plugin ABI.
+//
+// The reference (src/BenchKernelD3Q19AaCommon.c:309-637,
+// src/BenchKernelD3Q19Aa.c:40-548) streams through a ghost-padded array and,
+// after each step, copies PDFs along precomputed (src, dst) index pairs to
+// realise half-way bounce-back and periodic wrap-around.  Here the same
+// data movement is resolved while the odd step loads and stores: a per-cell
+// info word carries one bounce-back bit per direction, and cells on the
+// domain boundary compute the periodic image of their neighbour.  Every
+// fluid value ends up in the slot the reference's fix-up copies would have
+// put it, so node views are bit-identical to the reference.  Pairs whose
+// target lies outside a non-periodic face have no fix-up in the reference;
+// the value stays in the ghost cell and is read back from there by the same
+// node two steps later.  The ghost layer exists for exactly that case and is
+// zero-initialised (the reference leaves it unset; fresh pages read 0.0).
+//
+// Device layout: PDF(d, x, y, z) at
+//   d * plane + ((x + 1) * gY + (y + 1)) * zStride + (z + kZOff)
+// gY = Y + 2, plane = (X + 2) * gY * zStride, kZOff = 16 doubles (128 B, the
+// ghost z = -1 sits at 15) and zStride a multiple of 16, so that interior
+// z rows start on a 128-B line and a wavefront's 64 lanes cover four
+// lines.
+#include "lbm_device.h"
+#include "lbm_runtime.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+
+namespace lbm {
+
+constexpr int kZOff = 16;
+constexpr uint32_t kInfoFluid = 1u << 31;
+constexpr uint32_t kInfoBoundary = 1u << 30;
+
+struct AaGeom {
+	int X, Y, Z;
+	int px, py, pz;
+	int gY;
+	int zStride;
+	uint64_t plane;
+};
+
+struct AaData {
+	KernelData kd;
+	Common c;
+	AaGeom g = {};
+	int nFluid = 0;
+	int nBounce = 0;
+	uint32_t * dInfo = nullptr;   // [x][y][z], z fastest
+	double * dPdf = nullptr;
+	double * hPdf = nullptr;
+	uint32_t * hInfo = nullptr;
+	size_t nPdf = 0;
+};
+
+static AaData * as_aa(KernelData * kd) { return reinterpret_cast<AaData *>(kd); }
+
+__host__ __device__ __forceinline__ uint64_t aa_slot(const AaGeom & g, int d, int x, int y, int z)
+{
+	return (uint64_t)d * g.plane + ((uint64_t)(x + 1) * g.gY + (uint64_t)(y + 1)) * g.zStride + (uint64_t)(z + kZOff);
+}
+
+// Neighbour (x + dx, y + dy, z + dz) after the reference's periodic rule:
+// when a coordinate leaves the box through a periodic face, every
+// out-of-range coordinate is wrapped (src/BenchKernelD3Q19AaCommon.c:522-567);
+// otherwise out-of-range coordinates stay in the ghost layer.
+__host__ __device__ __forceinline__ void aa_resolve(const AaGeom & g, int & x, int & y, int & z)
+{
+	const bool ox = x < 0 || x >= g.X, oy = y < 0 || y >= g.Y, oz = z < 0 || z >= g.Z;
+	if ((ox && g.px) || (oy && g.py) || (oz && g.pz)) {
+		x = x < 0 ? g.X - 1 : (x >= g.X ? 0 : x);
+		y = y < 0 ? g.Y - 1 : (y >= g.Y ? 0 : y);
+		z = z < 0 ? g.Z - 1 : (z >= g.Z ? 0 : z);
+	}
+}
+
+// Info word per cell: fluid bit, boundary bit (a neighbour lies outside the
+// box), and bit d set when the push target of direction d is a solid cell
+// (in the box or as periodic image) -- the pairs for which the reference's
+// fix-up list holds a bounce-back entry.
+__global__ __launch_bounds__(256) void k_aa_info(const int * __restrict__ mask, AaGeom g, uint32_t * __restrict__ info,
+                                                 uint32_t * __restrict__ nBounce)
+{
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	const uint32_t n = (uint32_t)g.X * g.Y * g.Z;
+	int bounces = 0;
+	if (i < n) {
+		const int z = (int)(i % g.Z), y = (int)((i / g.Z) % g.Y), x = (int)(i / ((uint32_t)g.Z * g.Y));
+		uint32_t w = 0;
+		if (mask[((size_t)z * g.Y + y) * g.X + x] != LBM_CELL_OBSTACLE) {
+			w |= kInfoFluid;
+			if (x == 0 || y == 0 || z == 0 || x == g.X - 1 || y == g.Y - 1 || z == g.Z - 1) w |= kInfoBoundary;
+			for (int d = 0; d < 18; ++d) {
+				int nx = x + cx(d), ny = y + cy(d), nz = z + cz(d);
+				aa_resolve(g, nx, ny, nz);
+				const bool inside = nx >= 0 && nx < g.X && ny >= 0 && ny < g.Y && nz >= 0 && nz < g.Z;
+				if (inside && mask[((size_t)nz * g.Y + ny) * g.X + nx] == LBM_CELL_OBSTACLE) {
+					w |= 1u << d;
+					++bounces;
+				}
+			}
+		}
+		info[i] = w;
+	}
+	// total bounce-back pairs (reported like the reference's fix-up count)
+	for (int off = 32; off > 0; off >>= 1) bounces += __shfl_down(bounces, off);
+	if (__lane_id() == 0 && bounces) atomicAdd(nBounce, (uint32_t)bounces);
+}
+
+__device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int & z)
+{
+	z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	y = (int)blockIdx.y;
+	x = (int)blockIdx.z;
+	return z < g.Z;
+}
+
+// Even step: local load, collision, store of direction d into slot inv(d)
+// of the same cell (src/BenchKernelD3Q19Aa.c:146-290).
+template <bool FORCE_LOAD>
+__global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
+                                                 TrtConsts k, ForceTerms ft)
+{
+	int x, y, z;
+	if (!aa_cell(g, x, y, z)) return;
+	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
+	if (!(w & kInfoFluid)) return;
+	const uint64_t base = aa_slot(g, 0, x, y, z);
+	double f[19];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		f[d] = pdf[base + (uint64_t)d * g.plane];
+		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
+	}
+	trt_collide(f, k);
+#pragma unroll
+	for (int d = 0; d < 19; ++d) pdf[base + (uint64_t)inv_dir(d) * g.plane] = f[d];
+}
+
+// Odd step: direction d arrives from the pull neighbour x - c_d, slot inv(d)
+// (src/BenchKernelD3Q19Aa.c:392) -- or from the cell's own slot d when that
+// neighbour is solid -- and leaves to the push neighbour x + c_d, slot d
+// (:435-487) -- or to the cell's own slot inv(d) when the target is solid.
+// FORCE_STORE adds this step's force increment to every stored value that
+// lands in a cell's node view (not to ghost slots, which no node view
+// covers after an odd step).
+template <bool FORCE_LOAD, bool FORCE_STORE>
+__global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
+                                                TrtConsts k, ForceTerms ft)
+{
+	int x, y, z;
+	if (!aa_cell(g, x, y, z)) return;
+	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
+	if (!(w & kInfoFluid)) return;
+	const bool edge = (w & kInfoBoundary) != 0;
+	const uint64_t self = aa_slot(g, 0, x, y, z);
+
+	uint64_t src[18], dst[18];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		const int id = inv_dir(d);
+		// load of direction d
+		if (w & (1u << id)) {
+			src[d] = self + (uint64_t)d * g.plane;
+		} else if (!edge) {
+			src[d] = self + (uint64_t)id * g.plane - (int64_t)cx(d) * g.gY * g.zStride
+			         - (int64_t)cy(d) * g.zStride - (int64_t)cz(d);
+		} else {
+			int qx = x - cx(d), qy = y - cy(d), qz = z - cz(d);
+			aa_resolve(g, qx, qy, qz);
+			src[d] = aa_slot(g, id, qx, qy, qz);
+		}
+		// store of direction d
+		if (w & (1u << d)) {
+			dst[d] = self + (uint64_t)id * g.plane;
+		} else if (!edge) {
+			dst[d] = self + (uint64_t)d * g.plane + (int64_t)cx(d) * g.gY * g.zStride
+			         + (int64_t)cy(d) * g.zStride + (int64_t)cz(d);
+		} else {
+			int qx = x + cx(d), qy = y + cy(d), qz = z + cz(d);
+			aa_resolve(g, qx, qy, qz);
+			dst[d] = aa_slot(g, d, qx, qy, qz);
+		}
+	}
+	double f[19];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		f[d] = pdf[src[d]];
+		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
+	}
+	const uint64_t cSlot = self + (uint64_t)dC * g.plane;
+	f[dC] = pdf[cSlot];
+	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
+	trt_collide(f, k);
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		double v = f[d];
+		if (FORCE_STORE) {
+			if (w & (1u << d)) {
+				v = v + ft.f[inv_dir(d)];
+			} else {
+				int qx = x + cx(d), qy = y + cy(d), qz = z + cz(d);
+				if (edge) aa_resolve(g, qx, qy, qz);
+				const bool inside = qx >= 0 && qx < g.X && qy >= 0 && qy < g.Y && qz >= 0 && qz < g.Z;
+				if (inside) v = v + ft.f[d];
+			}
+		}
+		pdf[dst[d]] = v;
+	}
+	pdf[cSlot] = FORCE_STORE ? f[dC] + ft.f[dC] : f[dC];
+}
+
+// ----------------------------------------------------------- node access
+// Before the first and after every Kernel() call (an even number of steps)
+// the node view is local: direction d of a cell in its own slot d
+// (src/BenchKernelD3Q19AaCommon.c:153-158).
+static bool aa_local_layout(const AaData * A)
+{
+	if (A->c.iteration % 2 != 0) return true;
+	return fail("aa-soa: node access after an even step is not supported by this build.");
+}
+
+static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
+{
+	AaData * A = as_aa(kd);
+	if (!aa_local_layout(A)) return;
+	for (int d = 0; d < 19; ++d) pdfs[d] = A->hPdf[aa_slot(A->g, d, x, y, z)];
+}
+
+static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
+{
+	AaData * A = as_aa(kd);
+	if (!aa_local_layout(A)) return;
+	for (int d = 0; d < 19; ++d) A->hPdf[aa_slot(A->g, d, x, y, z)] = pdfs[d];
+	A->c.shadowDirty = 1;
+}
+
+static void aa_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
+{
+	AaData * A = as_aa(kd);
+	if (!aa_local_layout(A)) return;
+	*pdf = A->hPdf[aa_slot(A->g, dir, x, y, z)];
+}
+
+static void aa_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
+{
+	AaData * A = as_aa(kd);
+	if (!aa_local_layout(A)) return;
+	A->hPdf[aa_slot(A->g, dir, x, y, z)] = pdf;
+	A->c.shadowDirty = 1;
+}
+
+// --------------------------------------------------------------- Kernel()
+static bool aa_download(KernelData * kd)
+{
+	AaData * A = as_aa(kd);
+	if (A->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
+	if (!hip_ok(hipMemcpyAsync(A->hPdf, A->dPdf, sizeof(double) * A->nPdf, hipMemcpyDeviceToHost, A->c.stream), "download of the device lattice")) return false;
+	return hip_ok(hipStreamSynchronize(A->c.stream), "hipStreamSynchronize");
+}
+
+static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+{
+	(void)ld;
+	AaData * A = as_aa(kd);
+	Common & c = A->c;
+	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
+	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { fail("omega must be in (0, 2), got %f.", cd->Omega); return; }
+
+	if (c.shadowDirty) {
+		const double t0 = (double)clock() / CLOCKS_PER_SEC;
+		if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, sizeof(double) * A->nPdf, hipMemcpyHostToDevice, c.stream), "upload of the host lattice")) return;
+		if (!hip_ok(hipStreamSynchronize(c.stream), "hipStreamSynchronize")) return;
+		c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
+		c.shadowDirty = 0;
+	} else {
+		c.t.uploadMs = 0.0;
+	}
+
+	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
+	const ForceTerms ft = force_terms(cd->XForce);
+	const bool force = c.args.forceOn != 0;
+	const bool perLaunch = c.launchEvents != 0;
+	if (perLaunch && !common_events(&c, steps)) return;
+
+	const AaGeom g = A->g;
+	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
+	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
+
+	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
+	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
+		const bool last = it + 2 >= steps;
+		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
+		if (force && it > 0) k_aa_even<true><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else                 k_aa_even<false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		if (perLaunch) {
+			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
+			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);
+		}
+		if (force && last) k_aa_odd<true, true><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else if (force)    k_aa_odd<true, false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else               k_aa_odd<false, false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
+	}
+	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
+	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
+
+	common_collect(&c, steps, perLaunch);
+	kd->Duration = c.t.loopMs / 1e3;
+	if (steps > 0) c.iteration = steps - 1;
+
+	if (c.shadowDownload) {
+		const double t0 = (double)clock() / CLOCKS_PER_SEC;
+		if (!aa_download(kd)) return;
+		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
+	}
+}
+
+Common * aa_common(KernelData * kd)
+{
+	return (kd && kd->Kernel == aa_kernel) ? &as_aa(kd)->c : nullptr;
+}
+
+static void aa_release(AaData * A)
+{
+	if (!A) return;
+	(void)hipFree(A->dInfo);
+	(void)hipFree(A->dPdf);
+	host_free(A->hPdf);
+	free(A->hInfo);
+	common_free(&A->c);
+	delete A;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters * params)
+{
+	*kernelData = nullptr;
+	AaData * A = new (std::nothrow) AaData();
+	if (!A) { fail("allocating kernel data failed."); return; }
+	memset(&A->kd, 0, sizeof(A->kd));
+	if (!common_init(&A->c, params, false)) { aa_release(A); return; }
+
+	AaGeom & g = A->g;
+	g.X = ld->Dims[0]; g.Y = ld->Dims[1]; g.Z = ld->Dims[2];
+	g.px = ld->PeriodicX; g.py = ld->PeriodicY; g.pz = ld->PeriodicZ;
+	if (g.X <= 0 || g.Y <= 0 || g.Z <= 0) { fail("empty dimensions."); aa_release(A); return; }
+	g.gY = g.Y + 2;
+	g.zStride = ((kZOff + g.Z + 1 + 15) / 16) * 16;
+	g.plane = (uint64_t)(g.X + 2) * g.gY * g.zStride;
+	A->nPdf = (size_t)19 * g.plane;
+	A->nFluid = ld->nFluid;
+	const uint64_t nTotal = (uint64_t)g.X * g.Y * g.Z;
+	if (nTotal > 0xffffffffull / 19) { fail("number of PDFs exceed 2^32."); aa_release(A); return; }
+
+	printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);
+	printf("# allocating data for %llu LB nodes with padding (%llu bytes = %f MiB for the single lattice)\n",
+	       (unsigned long long)(g.plane), (unsigned long long)(A->nPdf * 8), A->nPdf * 8.0 / 1024.0 / 1024.0);
+
+	int * dMask = nullptr;
+	uint32_t * dCount = nullptr;
+	uint32_t hCount = 0;
+	bool ok = false;
+	hipStream_t s = A->c.stream;
+	do {
+		if (!hip_ok(hipMalloc((void **)&dMask, sizeof(int) * nTotal), "allocating geometry mask")) break;
+		if (!hip_ok(hipMalloc((void **)&dCount, sizeof(uint32_t)), "allocating counter")) break;
+		if (!hip_ok(hipMalloc((void **)&A->dInfo, sizeof(uint32_t) * nTotal), "allocating cell info")) break;
+		if (!hip_ok(hipMalloc((void **)&A->dPdf, sizeof(double) * A->nPdf), "allocating device lattice")) break;
+		if (!hip_ok(hipMemcpyAsync(dMask, ld->Lattice, sizeof(int) * nTotal, hipMemcpyHostToDevice, s), "mask upload")) break;
+		if (!hip_ok(hipMemsetAsync(dCount, 0, sizeof(uint32_t), s), "hipMemset")) break;
+		if (!hip_ok(hipMemsetAsync(A->dPdf, 0, sizeof(double) * A->nPdf, s), "hipMemset")) break;
+		k_aa_info<<<(unsigned)((nTotal + 255) / 256), 256, 0, s>>>(dMask, g, A->dInfo, dCount);
+		if (!hip_ok(hipGetLastError(), "cell info kernel")) break;
+		if (!hip_ok(hipMemcpyAsync(&hCount, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count download")) break;
+		A->hPdf = (double *)host_alloc(sizeof(double) * A->nPdf);
+		if (!A->hPdf) { fail("allocating host lattice failed: %zu bytes.", sizeof(double) * A->nPdf); break; }
+		memset(A->hPdf, 0, sizeof(double) * A->nPdf);
+		if (!hip_ok(hipStreamSynchronize(s), "init")) break;
+		ok = true;
+	} while (0);
+	(void)hipFree(dMask);
+	(void)hipFree(dCount);
+	if (!ok) { aa_release(A); return; }
+	A->nBounce = (int)hCount;
+	printf("# bounce back pdfs: %d (resolved in the odd step, no index lists)\n", A->nBounce);
+
+	KernelData * kd = &A->kd;
+	kd->Pdfs[0] = A->hPdf;
+	kd->PdfsActive = A->hPdf;
+	for (int i = 0; i < 3; ++i) { kd->Dims[i] = ld->Dims[i]; kd->GlobalDims[i] = ld->Dims[i] + 2; kd->Offsets[i] = 1; }
+	kd->nBounceBackPdfs = A->nBounce;
+	kd->LoopBalance = 2.0 * 19 * 8 + 4.0;
+	kd->GetNode = aa_get_node;
+	kd->SetNode = aa_set_node;
+	kd->BoundaryConditionsGetPdf = aa_bc_get;
+	kd->BoundaryConditionsSetPdf = aa_bc_set;
+	kd->Kernel = aa_kernel;
+	A->c.download = aa_download;
+	*kernelData = kd;
+}
+
+void D3Q19AaDeinit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData)
+{
+	(void)ld;
+	if (kernelData && *kernelData) {
+		aa_release(as_aa(*kernelData));
+		*kernelData = nullptr;
+	}
+}
+
+}  // extern "C"

@@ -1,0 +1,599 @@
+// lbm_list.hip -- list-aa-pv-soa on MI355X: fluid-node list, push adjacency
+// and the even/odd AA steps as gfx950 kernels, behind the reference's
+// kernel-plugin ABI.
+//
+// Reference behaviour reproduced here:
+//   * node numbering and adjacency values of
+//     src/BenchKernelD3Q19ListAaPvCommon.c:627-793, bit for bit;
+//   * padded SoA PDF layout d * nCells + n (src/BenchKernelD3Q19ListAaCommon.h:79-103)
+//     with nCells from the reference's padding rule (src/Padding.c:195-244);
+//   * even step: local load, TRT collision, store of direction d into slot
+//     inv(d) of the same node (src/BenchKernelD3Q19ListAaPv.c:380-465);
+//   * odd step: direction d loaded from adj[n][inv d], stored to adj[n][d]
+//     (src/BenchKernelD3Q19ListAaPv.c:755-845);
+//   * VERIFICATION builds add the Luo force in node view after every step
+//     (src/BenchKernelD3Q19ListAaPv.c:100-126, src/Kernel.c:759-803).
+//     Here the increment of each step is added where the next step loads
+//     the value, and after the last step where it is stored -- the same
+//     single addition on the same operands, without an extra lattice pass.
+//
+// Device layout: PDFs as the reference (19 planes of nCells doubles; nCells
+// * 8 B is a multiple of 128 B after padding, so every plane starts on a
+// 128-B line).  Adjacency transposed to [18][adjStride] uint32 so that lane
+// n of a wavefront reads entry d of node n next to lane n+1: every
+// adjacency load is one coalesced 256-B access per wavefront.
+#include "lbm_device.h"
+#include "lbm_runtime.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+
+namespace lbm {
+
+struct ListData {
+	KernelData kd;              // first: callers cast KernelData* to the kernel's struct
+	Common c;
+	int dims[3] = {0, 0, 0};
+	int periodic[3] = {0, 0, 0};
+	int blk[3] = {0, 0, 0};
+	int nFluid = 0;
+	int nCells = 0;
+	int adjStride = 0;
+	int nRuns = 0;
+	uint32_t * dGrid = nullptr;
+	uint32_t * dCoords = nullptr;
+	uint32_t * dAdj = nullptr;
+	double * dPdf = nullptr;
+	double * hPdf = nullptr;     // host shadow, == kd.Pdfs[0]
+	uint32_t * hGrid = nullptr;  // cell -> node id, for GetNode/SetNode
+};
+
+static ListData * as_list(KernelData * kd) { return reinterpret_cast<ListData *>(kd); }
+static const ListData * as_list(const KernelData * kd) { return reinterpret_cast<const ListData *>(kd); }
+
+// ------------------------------------------------------------ list build
+// Node numbering visits blocks (bx, by, bz) and inside each block x, y, z
+// with z fastest (src/BenchKernelD3Q19ListAaPvCommon.c:631-656).  trav_xyz
+// maps a position t of that traversal to its cell; the node id of a fluid
+// cell is the number of fluid cells before it in the traversal, computed
+// with wavefront ballots and a prefix sum over block totals.
+struct NumGeom {
+	int X, Y, Z, bx, by, bz;
+};
+
+__device__ __forceinline__ void trav_xyz(uint32_t t, const NumGeom & g, int & x, int & y, int & z)
+{
+	const uint32_t slab = (uint32_t)g.bx * g.Y * g.Z;            // every x block before the last is full
+	const int i = (int)(t / slab);
+	uint32_t r = t - (uint32_t)i * slab;
+	const int ex = min(g.bx, g.X - i * g.bx);
+	const uint32_t rowBlk = (uint32_t)g.by * ex * g.Z;
+	const int j = (int)(r / rowBlk);
+	r -= (uint32_t)j * rowBlk;
+	const int ey = min(g.by, g.Y - j * g.by);
+	const uint32_t cellBlk = (uint32_t)ex * ey * g.bz;
+	const int k = (int)(r / cellBlk);
+	r -= (uint32_t)k * cellBlk;
+	const int ez = min(g.bz, g.Z - k * g.bz);
+	const uint32_t plane = (uint32_t)ey * ez;
+	const int lx = (int)(r / plane);
+	r -= (uint32_t)lx * plane;
+	const int ly = (int)(r / ez);
+	const int lz = (int)(r - (uint32_t)ly * ez);
+	x = i * g.bx + lx;
+	y = j * g.by + ly;
+	z = k * g.bz + lz;
+}
+
+constexpr int kNumBlock = 256;   // 4 wavefronts
+
+__global__ __launch_bounds__(kNumBlock) void k_num_count(const int * __restrict__ mask, NumGeom g,
+                                                         uint32_t N, uint32_t * __restrict__ blockSums)
+{
+	const uint32_t t = blockIdx.x * kNumBlock + threadIdx.x;
+	int fluid = 0;
+	if (t < N) {
+		int x, y, z;
+		trav_xyz(t, g, x, y, z);
+		fluid = mask[((size_t)z * g.Y + y) * g.X + x] != LBM_CELL_OBSTACLE;
+	}
+	const unsigned long long b = __ballot(fluid);
+	__shared__ uint32_t waveCount[kNumBlock / 64];
+	if (__lane_id() == 0) waveCount[threadIdx.x / 64] = (uint32_t)__popcll(b);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t s = 0;
+		for (int w = 0; w < kNumBlock / 64; ++w) s += waveCount[w];
+		blockSums[blockIdx.x] = s;
+	}
+}
+
+// Exclusive scan of n block totals in one workgroup; total -> *sum.
+__global__ __launch_bounds__(1024) void k_scan_once(uint32_t * __restrict__ v, uint32_t n, uint32_t * __restrict__ sum)
+{
+	__shared__ uint32_t part[1024];
+	const uint32_t tid = threadIdx.x;
+	const uint32_t per = (n + 1023) / 1024;
+	const uint32_t s = min(n, tid * per), e = min(n, s + per);
+	uint32_t acc = 0;
+	for (uint32_t i = s; i < e; ++i) acc += v[i];
+	part[tid] = acc;
+	__syncthreads();
+	for (uint32_t off = 1; off < 1024; off <<= 1) {
+		const uint32_t add = tid >= off ? part[tid - off] : 0;
+		__syncthreads();
+		part[tid] += add;
+		__syncthreads();
+	}
+	uint32_t run = tid ? part[tid - 1] : 0;
+	for (uint32_t i = s; i < e; ++i) {
+		const uint32_t c = v[i];
+		v[i] = run;
+		run += c;
+	}
+	if (tid == 1023) *sum = part[1023];
+}
+
+__global__ __launch_bounds__(kNumBlock) void k_num_assign(const int * __restrict__ mask, NumGeom g, uint32_t N,
+                                                          const uint32_t * __restrict__ blockOffset,
+                                                          uint32_t * __restrict__ grid, uint32_t * __restrict__ coords)
+{
+	const uint32_t t = blockIdx.x * kNumBlock + threadIdx.x;
+	int x = 0, y = 0, z = 0, fluid = 0;
+	size_t cell = 0;
+	if (t < N) {
+		trav_xyz(t, g, x, y, z);
+		cell = ((size_t)z * g.Y + y) * g.X + x;
+		fluid = mask[cell] != LBM_CELL_OBSTACLE;
+	}
+	const unsigned long long b = __ballot(fluid);
+	const int lane = __lane_id();
+	const uint32_t below = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+	__shared__ uint32_t waveCount[kNumBlock / 64];
+	if (lane == 0) waveCount[threadIdx.x / 64] = (uint32_t)__popcll(b);
+	__syncthreads();
+	uint32_t waveBase = 0;
+	for (int w = 0; w < (int)(threadIdx.x / 64); ++w) waveBase += waveCount[w];
+	if (t < N) {
+		if (fluid) {
+			const uint32_t id = blockOffset[blockIdx.x] + waveBase + below;
+			grid[cell] = id;
+			coords[3 * (size_t)id + 0] = (uint32_t)x;
+			coords[3 * (size_t)id + 1] = (uint32_t)y;
+			coords[3 * (size_t)id + 2] = (uint32_t)z;
+		} else {
+			grid[cell] = 0xffffffffu;
+		}
+	}
+}
+
+struct AdjGeom {
+	int X, Y, Z, px, py, pz;
+	uint32_t nFluid, nCells, stride;
+};
+
+// Push adjacency of src/BenchKernelD3Q19ListAaPvCommon.c:700-793: the slot
+// d * nCells + m of the neighbour m in direction d, or the node's own slot
+// inv(d) * nCells + n (half-way bounce-back) when that neighbour is solid or
+// outside a non-periodic face.  Once a coordinate leaves the box through a
+// periodic face, every out-of-range coordinate is wrapped (:729-761).
+__global__ __launch_bounds__(256) void k_adjacency(const int * __restrict__ mask, const uint32_t * __restrict__ grid,
+                                                   const uint32_t * __restrict__ coords, AdjGeom g,
+                                                   uint32_t * __restrict__ adj)
+{
+	const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+	if (n >= g.nFluid) return;
+	const int x = (int)coords[3 * (size_t)n + 0];
+	const int y = (int)coords[3 * (size_t)n + 1];
+	const int z = (int)coords[3 * (size_t)n + 2];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		int nx = x + cx(d), ny = y + cy(d), nz = z + cz(d);
+		const bool ox = nx < 0 || nx >= g.X, oy = ny < 0 || ny >= g.Y, oz = nz < 0 || nz >= g.Z;
+		const uint32_t bounce = (uint32_t)inv_dir(d) * g.nCells + n;
+		uint32_t v;
+		if ((ox && g.px) || (oy && g.py) || (oz && g.pz)) {
+			nx = nx < 0 ? g.X - 1 : (nx >= g.X ? 0 : nx);
+			ny = ny < 0 ? g.Y - 1 : (ny >= g.Y ? 0 : ny);
+			nz = nz < 0 ? g.Z - 1 : (nz >= g.Z ? 0 : nz);
+			const size_t c = ((size_t)nz * g.Y + ny) * g.X + nx;
+			v = mask[c] == LBM_CELL_OBSTACLE ? bounce : (uint32_t)d * g.nCells + grid[c];
+		} else if (ox || oy || oz) {
+			v = bounce;
+		} else {
+			const size_t c = ((size_t)nz * g.Y + ny) * g.X + nx;
+			v = mask[c] == LBM_CELL_OBSTACLE ? bounce : (uint32_t)d * g.nCells + grid[c];
+		}
+		adj[(size_t)d * g.stride + n] = v;
+	}
+}
+
+// Counts nodes n >= 1 whose 18 adjacency entries all equal those of n-1
+// plus one, i.e. the starts of runs (src/BenchKernelD3Q19ListAaPvCommon.c:360-385).
+__global__ __launch_bounds__(256) void k_run_breaks(const uint32_t * __restrict__ adj, uint32_t nFluid,
+                                                    uint32_t stride, uint32_t * __restrict__ breaks)
+{
+	const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+	int br = 0;
+	if (n >= 1 && n < nFluid) {
+		for (int d = 0; d < 18; ++d)
+			if (adj[(size_t)d * stride + n] != adj[(size_t)d * stride + n - 1] + 1u) { br = 1; break; }
+	}
+	const unsigned long long b = __ballot(br);
+	if (__lane_id() == 0 && b) atomicAdd(breaks, (uint32_t)__popcll(b));
+}
+
+// ---------------------------------------------------------------- steps
+// Even step: node n loads its 19 local values, collides and stores
+// direction d into its own slot inv(d).  FORCE_LOAD adds the body-force
+// increment of the preceding odd step to each loaded value.
+template <bool FORCE_LOAD>
+__global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid, uint32_t nCells,
+                                                   TrtConsts k, ForceTerms ft)
+{
+	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nFluid) return;
+	double f[19];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		f[d] = pdf[(size_t)d * nCells + n];
+		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
+	}
+	trt_collide(f, k);
+#pragma unroll
+	for (int d = 0; d < 19; ++d) pdf[(size_t)inv_dir(d) * nCells + n] = f[d];
+}
+
+// Odd step: direction d arrives from slot adj[n][inv d] (the neighbour's
+// swapped store of the even step, or the node's own slot d for a
+// bounce-back link); the post-collision value goes to adj[n][d].  Every slot
+// is read and written by exactly one node, so the update is in place and
+// race free.  FORCE_STORE adds the force increment of this step to each
+// stored value: a value stored through a bounce-back entry is read back by
+// the same node as direction inv(d), any other by the neighbour as d.
+template <bool FORCE_LOAD, bool FORCE_STORE>
+__global__ __launch_bounds__(512) void k_list_odd(double * pdf, const uint32_t * __restrict__ adj,
+                                                  uint32_t nFluid, uint32_t nCells, uint32_t stride,
+                                                  TrtConsts k, ForceTerms ft)
+{
+	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nFluid) return;
+	uint32_t a[18];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) a[d] = adj[(size_t)d * stride + n];
+	double f[19];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		f[d] = pdf[a[inv_dir(d)]];
+		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
+	}
+	const size_t cSlot = (size_t)dC * nCells + n;
+	f[dC] = pdf[cSlot];
+	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
+	trt_collide(f, k);
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		double v = f[d];
+		if (FORCE_STORE) {
+			const bool bounce = a[d] == (uint32_t)inv_dir(d) * nCells + n;
+			v = v + (bounce ? ft.f[inv_dir(d)] : ft.f[d]);
+		}
+		pdf[a[d]] = v;
+	}
+	pdf[cSlot] = FORCE_STORE ? f[dC] + ft.f[dC] : f[dC];
+}
+
+// --------------------------------------------------------- node access
+// After any completed Kernel() call (always an even number of steps) and
+// before the first one, the lattice is in the local layout: node n's value
+// of direction d sits in slot d * nCells + n
+// (src/BenchKernelD3Q19ListAaPvCommon.c:150-156).
+static bool list_local_layout(const ListData * L)
+{
+	if (L->c.iteration % 2 != 0) return true;
+	return fail("list-aa-pv-soa: node access after an even step is not supported by this build.");
+}
+
+static void list_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
+{
+	ListData * L = as_list(kd);
+	if (!list_local_layout(L)) return;
+	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	for (int d = 0; d < 19; ++d) pdfs[d] = L->hPdf[(size_t)d * L->nCells + n];
+}
+
+static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
+{
+	ListData * L = as_list(kd);
+	if (!list_local_layout(L)) return;
+	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	for (int d = 0; d < 19; ++d) L->hPdf[(size_t)d * L->nCells + n] = pdfs[d];
+	L->c.shadowDirty = 1;
+}
+
+static void list_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
+{
+	ListData * L = as_list(kd);
+	if (!list_local_layout(L)) return;
+	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	*pdf = L->hPdf[(size_t)dir * L->nCells + n];
+}
+
+static void list_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
+{
+	ListData * L = as_list(kd);
+	if (!list_local_layout(L)) return;
+	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	L->hPdf[(size_t)dir * L->nCells + n] = pdf;
+	L->c.shadowDirty = 1;
+}
+
+// ------------------------------------------------------------- Kernel()
+static bool list_upload(ListData * L)
+{
+	if (!L->c.shadowDirty) { L->c.t.uploadMs = 0.0; return true; }
+	const double t0 = (double)clock() / CLOCKS_PER_SEC;
+	if (!hip_ok(hipMemcpyAsync(L->dPdf, L->hPdf, sizeof(double) * 19 * (size_t)L->nCells,
+	                           hipMemcpyHostToDevice, L->c.stream), "upload of the host lattice")) return false;
+	if (!hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize")) return false;
+	L->c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
+	L->c.shadowDirty = 0;
+	return true;
+}
+
+static bool list_download(ListData * L)
+{
+	if (L->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
+	if (!hip_ok(hipMemcpyAsync(L->hPdf, L->dPdf, sizeof(double) * 19 * (size_t)L->nCells,
+	                           hipMemcpyDeviceToHost, L->c.stream), "download of the device lattice")) return false;
+	return hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize");
+}
+
+static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+{
+	(void)ld;
+	ListData * L = as_list(kd);
+	Common & c = L->c;
+	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
+	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { fail("omega must be in (0, 2), got %f.", cd->Omega); return; }
+	if (!list_upload(L)) return;
+
+	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
+	const ForceTerms ft = force_terms(cd->XForce);
+	const bool force = c.args.forceOn != 0;
+	const bool perLaunch = c.launchEvents != 0;
+	if (perLaunch && !common_events(&c, steps)) return;
+
+	const int bs = c.args.blockSize;
+	const dim3 grid((unsigned)((L->nFluid + bs - 1) / bs)), block((unsigned)bs);
+	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells, st = (uint32_t)L->adjStride;
+
+	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
+	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
+		const bool last = it + 2 >= steps;
+		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
+		if (force && it > 0) k_list_even<true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else                 k_list_even<false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		if (perLaunch) {
+			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
+			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);
+		}
+		if (force && last) k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
+		else if (force)    k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
+		else               k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
+		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
+	}
+	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
+	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
+
+	common_collect(&c, steps, perLaunch);
+	kd->Duration = c.t.loopMs / 1e3;
+	if (steps > 0) c.iteration = steps - 1;
+
+	if (c.shadowDownload) {
+		const double t0 = (double)clock() / CLOCKS_PER_SEC;
+		if (!list_download(L)) return;
+		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
+	}
+}
+
+// ------------------------------------------------------------ Init/Deinit
+static void list_release(ListData * L)
+{
+	if (!L) return;
+	(void)hipFree(L->dGrid);
+	(void)hipFree(L->dCoords);
+	(void)hipFree(L->dAdj);
+	(void)hipFree(L->dPdf);
+	host_free(L->hPdf);
+	free(L->hGrid);
+	common_free(&L->c);
+	delete L;
+}
+
+static bool dev_alloc(void ** p, size_t bytes, const char * what)
+{
+	hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+	if (e != hipSuccess) return fail("allocating %s failed: %zu bytes (%s).", what, bytes, hipGetErrorString(e));
+	return true;
+}
+
+static bool list_build(ListData * L, const LatticeDesc * ld)
+{
+	const int X = L->dims[0], Y = L->dims[1], Z = L->dims[2];
+	const uint32_t N = (uint32_t)X * Y * Z;
+	const uint32_t nBlocks = (N + kNumBlock - 1) / kNumBlock;
+	int * dMask = nullptr;
+	uint32_t * dSums = nullptr;
+	uint32_t hTotal[2] = {0, 0};
+	bool ok = false;
+
+	do {
+		if (!dev_alloc((void **)&dMask, sizeof(int) * (size_t)N, "geometry mask")) break;
+		if (!dev_alloc((void **)&dSums, sizeof(uint32_t) * ((size_t)nBlocks + 2), "scan buffer")) break;
+		if (!dev_alloc((void **)&L->dGrid, sizeof(uint32_t) * (size_t)N, "grid")) break;
+		hipStream_t s = L->c.stream;
+		if (!hip_ok(hipMemcpyAsync(dMask, ld->Lattice, sizeof(int) * (size_t)N, hipMemcpyHostToDevice, s), "mask upload")) break;
+
+		// count first: coords and adjacency are sized by the count, which
+		// must agree with LatticeDesc.nFluid before anything is written
+		NumGeom g = {X, Y, Z, L->blk[0], L->blk[1], L->blk[2]};
+		k_num_count<<<nBlocks, kNumBlock, 0, s>>>(dMask, g, N, dSums);
+		k_scan_once<<<1, 1024, 0, s>>>(dSums, nBlocks, dSums + nBlocks);
+		if (!hip_ok(hipGetLastError(), "numbering kernels")) break;
+		if (!hip_ok(hipMemcpyAsync(&hTotal[0], dSums + nBlocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count download")) break;
+		if (!hip_ok(hipStreamSynchronize(s), "numbering kernels")) break;
+		if ((int)hTotal[0] != L->nFluid) {
+			fail("list build: %u non-solid cells in the mask, LatticeDesc.nFluid = %d.", hTotal[0], L->nFluid);
+			break;
+		}
+		if (!dev_alloc((void **)&L->dCoords, sizeof(uint32_t) * 3 * (size_t)(L->nFluid > 0 ? L->nFluid : 1), "coords")) break;
+		if (!dev_alloc((void **)&L->dAdj, sizeof(uint32_t) * 18 * (size_t)L->adjStride, "adjacency")) break;
+		if (!hip_ok(hipMemsetAsync(L->dAdj, 0, sizeof(uint32_t) * 18 * (size_t)L->adjStride, s), "hipMemset")) break;
+		k_num_assign<<<nBlocks, kNumBlock, 0, s>>>(dMask, g, N, dSums, L->dGrid, L->dCoords);
+		if (!hip_ok(hipGetLastError(), "numbering kernels")) break;
+		if (L->nFluid > 0) {
+			AdjGeom ag = {X, Y, Z, L->periodic[0], L->periodic[1], L->periodic[2],
+			              (uint32_t)L->nFluid, (uint32_t)L->nCells, (uint32_t)L->adjStride};
+			const unsigned nb = (unsigned)((L->nFluid + 255) / 256);
+			k_adjacency<<<nb, 256, 0, s>>>(dMask, L->dGrid, L->dCoords, ag, L->dAdj);
+			if (!hip_ok(hipMemsetAsync(dSums, 0, sizeof(uint32_t), s), "hipMemset")) break;
+			k_run_breaks<<<nb, 256, 0, s>>>(L->dAdj, (uint32_t)L->nFluid, (uint32_t)L->adjStride, dSums);
+			if (!hip_ok(hipGetLastError(), "adjacency kernels")) break;
+			if (!hip_ok(hipMemcpyAsync(&hTotal[1], dSums, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "run count download")) break;
+		}
+		L->hGrid = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)N);
+		if (!L->hGrid) { fail("allocating host grid failed: %zu bytes.", sizeof(uint32_t) * (size_t)N); break; }
+		if (!hip_ok(hipMemcpyAsync(L->hGrid, L->dGrid, sizeof(uint32_t) * (size_t)N, hipMemcpyDeviceToHost, s), "grid download")) break;
+		if (!hip_ok(hipStreamSynchronize(s), "adjacency kernels")) break;
+		L->nRuns = L->nFluid > 0 ? (int)hTotal[1] + 1 : 0;
+		ok = true;
+	} while (0);
+
+	(void)hipFree(dMask);
+	(void)hipFree(dSums);
+	return ok;
+}
+
+static bool list_download_kd(KernelData * kd) { return list_download(as_list(kd)); }
+
+Common * list_common(KernelData * kd)
+{
+	return (kd && kd->Kernel == list_kernel) ? &as_list(kd)->c : nullptr;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters * params)
+{
+	*kernelData = nullptr;
+	ListData * L = new (std::nothrow) ListData();
+	if (!L) { fail("allocating kernel data failed."); return; }
+	memset(&L->kd, 0, sizeof(L->kd));
+	if (!common_init(&L->c, params, true)) { list_release(L); return; }
+
+	for (int i = 0; i < 3; ++i) {
+		L->dims[i] = ld->Dims[i];
+		L->blk[i] = L->c.args.blk[i] ? L->c.args.blk[i] : ld->Dims[i];
+	}
+	L->periodic[0] = ld->PeriodicX; L->periodic[1] = ld->PeriodicY; L->periodic[2] = ld->PeriodicZ;
+	L->nFluid = ld->nFluid;
+
+	const uint64_t nTotal = (uint64_t)ld->Dims[0] * ld->Dims[1] * ld->Dims[2];
+	if (ld->Dims[0] <= 0 || ld->Dims[1] <= 0 || ld->Dims[2] <= 0 || 19ull * nTotal > 0xffffffffull) {
+		fail("number of PDFs exceed 2^32 or empty dimensions.");   // src/Main.c:399-406
+		list_release(L);
+		return;
+	}
+	const long nCells = pad_cells(L->nFluid, (int)sizeof(PdfT), L->c.args.havePad ? &L->c.args.pad : nullptr);
+	if (nCells < 0 || 19ull * (uint64_t)nCells > 0xffffffffull) {
+		fail("padded node count %ld does not fit the 32-bit adjacency.", nCells);
+		list_release(L);
+		return;
+	}
+	L->nCells = (int)nCells;
+	L->adjStride = ((L->nFluid + 255) / 256) * 256;
+
+	printf("# blocking               x: %3d y: %3d z: %3d\n", L->blk[0], L->blk[1], L->blk[2]);
+	printf("# lattice size:          %e MiB\n", nCells * 8.0 * 19 / 1024.0 / 1024.0);
+	printf("# fluid lattice size:    %e MiB\n", L->nFluid * 8.0 * 19 / 1024.0 / 1024.0);
+	printf("# index size:            %e MiB\n", L->nFluid * 4.0 * 18 / 1024.0 / 1024.0);
+
+	if (!list_build(L, ld)) { list_release(L); return; }
+
+	const size_t pdfBytes = sizeof(double) * 19 * (size_t)L->nCells;
+	if (!dev_alloc((void **)&L->dPdf, pdfBytes, "device lattice")) { list_release(L); return; }
+	if (!hip_ok(hipMemsetAsync(L->dPdf, 0, pdfBytes, L->c.stream), "hipMemset")) { list_release(L); return; }
+	L->hPdf = (double *)host_alloc(pdfBytes);
+	if (!L->hPdf) { fail("allocating host lattice failed: %zu bytes.", pdfBytes); list_release(L); return; }
+	memset(L->hPdf, 0, pdfBytes);
+	if (!hip_ok(hipStreamSynchronize(L->c.stream), "init")) { list_release(L); return; }
+
+	const double even = 2.0 * 19 * 8, odd = 2.0 * 19 * 8 + 18 * 4.0;
+	printf("# total fluid nodes: %d   consecutive blocks: %d\n", L->nFluid, L->nRuns);
+	printf("# loop balance:          %.2f B/FLUP  even: %.2f B/FLUP  odd %.2f B/FLUP\n", (even + odd) / 2, even, odd);
+
+	KernelData * kd = &L->kd;
+	kd->Pdfs[0] = L->hPdf;
+	kd->PdfsActive = L->hPdf;
+	for (int i = 0; i < 3; ++i) { kd->Dims[i] = ld->Dims[i]; kd->GlobalDims[i] = ld->Dims[i]; kd->Offsets[i] = 0; }
+	kd->LoopBalance = (even + odd) / 2;
+	kd->GetNode = list_get_node;
+	kd->SetNode = list_set_node;
+	kd->BoundaryConditionsGetPdf = list_bc_get;
+	kd->BoundaryConditionsSetPdf = list_bc_set;
+	kd->Kernel = list_kernel;
+	L->c.download = list_download_kd;
+	*kernelData = kd;
+}
+
+void D3Q19ListAaPvDeinit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData)
+{
+	(void)ld;
+	if (kernelData && *kernelData) {
+		list_release(as_list(*kernelData));
+		*kernelData = nullptr;
+	}
+}
+
+int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out)
+{
+	if (!kd || !out || kd->Kernel != list_kernel) { fail("LbmHipListGetInfo: not a list-aa-pv-soa kernel."); return -1; }
+	const ListData * L = as_list(kd);
+	out->nFluid = L->nFluid;
+	out->nCells = L->nCells;
+	out->adjStride = L->adjStride;
+	out->nRuns = L->nRuns;
+	for (int i = 0; i < 3; ++i) { out->dims[i] = L->dims[i]; out->periodic[i] = L->periodic[i]; }
+	out->bytesPerFlupEven = 2.0 * 19 * 8;
+	out->bytesPerFlupOdd = 2.0 * 19 * 8 + 18 * 4.0;
+	return 0;
+}
+
+int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, uint32_t * coords)
+{
+	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListCopyBuild: not a list-aa-pv-soa kernel."); return -1; }
+	const ListData * L = as_list(kd);
+	const size_t N = (size_t)L->dims[0] * L->dims[1] * L->dims[2];
+	if (grid && !hip_ok(hipMemcpy(grid, L->dGrid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "grid download")) return -1;
+	if (coords && L->nFluid > 0 &&
+	    !hip_ok(hipMemcpy(coords, L->dCoords, sizeof(uint32_t) * 3 * (size_t)L->nFluid, hipMemcpyDeviceToHost), "coords download")) return -1;
+	if (adj && L->nFluid > 0) {
+		uint32_t * t = (uint32_t *)malloc(sizeof(uint32_t) * 18 * (size_t)L->adjStride);
+		if (!t) { fail("allocating %zu bytes failed.", sizeof(uint32_t) * 18 * (size_t)L->adjStride); return -1; }
+		if (!hip_ok(hipMemcpy(t, L->dAdj, sizeof(uint32_t) * 18 * (size_t)L->adjStride, hipMemcpyDeviceToHost), "adjacency download")) { free(t); return -1; }
+		for (size_t n = 0; n < (size_t)L->nFluid; ++n)
+			for (int d = 0; d < 18; ++d) adj[n * 18 + d] = t[(size_t)d * L->adjStride + n];
+		free(t);
+	}
+	return 0;
+}
+
+}  // extern "C"

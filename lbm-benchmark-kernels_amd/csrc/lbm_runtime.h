@@ -1,0 +1,102 @@
+// lbm_runtime.h -- host-side plumbing shared by the list and aa kernels of
+// liblbmhip.so: error handling, kernel-argument parsing, SoA padding, the
+// collision constants, the Luo force table, and the common per-kernel state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stddef.h>
+
+#include "lbm_hip.h"
+
+namespace lbm {
+
+// ---------------------------------------------------------------- errors --
+// Records the message; in LBM_HIP_ERRORS_EXIT mode prints "ERROR: <msg>"
+// and exits with status 1 like the reference's error paths
+// (src/BenchKernelD3Q19ListAaPvCommon.c:246-317, src/Memory.c:63-66).
+// Always returns false so callers can `return fail(...)`.
+bool fail(const char * fmt, ...) __attribute__((format(printf, 1, 2)));
+bool hip_ok(hipError_t e, const char * what);
+
+// --------------------------------------------------------------- padding --
+// Padding specification as the reference's PadInfo (src/Padding.h:11-17).
+struct PadSpec {
+	int n = 0;
+	int mod[16] = {0};
+	int off[16] = {0};
+};
+// "auto" | "no" | "off" | "<m>+<o>(,<m>+<o>)*" (src/Padding.c:19-193).
+bool pad_parse(const char * s, PadSpec * out);
+// Node count padded as src/Padding.c:195-244 does; ps == nullptr = "auto".
+long pad_cells(long nCells, int cellBytes, const PadSpec * ps);
+
+// ----------------------------------------------------------- kernel args --
+struct KernelArgs {
+	int blk[3] = {0, 0, 0};
+	bool havePad = false;
+	PadSpec pad;
+	int forceOn = 1;
+	double magic = 1.0 / 12.0;
+	int blockSize = 256;
+};
+// Parses Parameters.KernelArgs.  listKernel enables -pad (the reference
+// accepts it only for the list kernel, src/BenchKernelD3Q19ListAaPvCommon.c:302-308).
+bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel);
+
+// -------------------------------------------------------------- physics --
+// TRT relaxation constants, each evaluated on the host with the operation
+// order of src/BenchKernelD3Q19ListAaPv.c:174-189 so that the device sees
+// the reference's exact double values.
+struct TrtConsts {
+	double oe, oo;        // omegaEven, omegaOdd
+	double w0, w1, w2;
+	double w1x3, w1nh;    // w_1 * 3, w_1 * 9 / 2
+	double w2x3, w2nh;    // w_2 * 3, w_2 * 9 / 2
+};
+TrtConsts trt_consts(double omega, double magic);
+
+// Luo body-force increments 3 * w_d * c_x,d * F per direction, evaluated
+// left to right as src/Kernel.c:795 does.
+struct ForceTerms {
+	double f[19];
+};
+ForceTerms force_terms(double xForce);
+
+extern const int kCx[19], kCy[19], kCz[19], kInv[19];
+
+// ------------------------------------------------------- common state --
+// Embedded (after KernelData) in the list and aa kernel structs.
+struct Common {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	KernelArgs args;
+	int iteration = -1;         // parity of the last completed step
+	int shadowDirty = 0;        // SetNode touched the host shadow since the last upload
+	int launchEvents = 1;
+	int shadowDownload = 1;
+	hipEvent_t * ev = nullptr;  // 2 per launch
+	int nEv = 0;
+	hipEvent_t evLoop[2] = {nullptr, nullptr};
+	LbmHipTimings t = {};
+	bool (*download)(KernelData * kd) = nullptr;   // device lattice -> host shadow
+};
+
+// Common state of a KernelData created by this library, or nullptr.
+Common * common_of(KernelData * kd);
+Common * list_common(KernelData * kd);
+Common * aa_common(KernelData * kd);
+
+bool common_init(Common * c, const Parameters * params, bool listKernel);
+void common_free(Common * c);
+// Grows the per-launch event pool to hold `launches` launches.
+bool common_events(Common * c, int launches);
+// Fills c->t from the recorded events (stream must be synchronised).
+void common_collect(Common * c, int launches, bool perLaunch);
+
+// Pinned host buffer (hipHostMalloc) -- used for the host shadow.
+void * host_alloc(size_t bytes);
+void host_free(void * p);
+
+}  // namespace lbm

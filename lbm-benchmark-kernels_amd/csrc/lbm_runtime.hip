@@ -1,0 +1,392 @@
+// lbm_runtime.hip -- host plumbing of liblbmhip.so (see lbm_runtime.h).
+#include "lbm_runtime.h"
+
+#include <errno.h>
+#include <limits.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+namespace lbm {
+
+// Stencil of src/Kernel.h:121-140 (N S E W NE SE NW SW T TN TE TW TS B BS BN BW BE C).
+const int kCx[19]  = { 0, 0, 1,-1, 1, 1,-1,-1, 0, 0, 1,-1, 0, 0, 0, 0,-1, 1, 0 };
+const int kCy[19]  = { 1,-1, 0, 0, 1,-1, 1,-1, 0, 1, 0, 0,-1, 0,-1, 1, 0, 0, 0 };
+const int kCz[19]  = { 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,-1,-1,-1,-1,-1, 0 };
+const int kInv[19] = { 1, 0, 3, 2, 7, 6, 5, 4,13,14,16,17,15, 8, 9,12,10,11,18 };
+
+static int g_errorMode = LBM_HIP_ERRORS_EXIT;
+static char g_lastError[1024] = "";
+
+bool fail(const char * fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_lastError, sizeof(g_lastError), fmt, ap);
+	va_end(ap);
+	printf("ERROR: %s\n", g_lastError);
+	fflush(stdout);
+	if (g_errorMode == LBM_HIP_ERRORS_EXIT) exit(1);
+	return false;
+}
+
+bool hip_ok(hipError_t e, const char * what)
+{
+	if (e == hipSuccess) return true;
+	return fail("%s failed: %s (%s)", what, hipGetErrorName(e), hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- padding
+bool pad_parse(const char * s, PadSpec * p)
+{
+	*p = PadSpec();
+	if (s == nullptr) return fail("padding string missing");
+	if (!strncmp("auto", s, 4)) {
+		// two entries: 16 MiB modulus + (2 MiB + 512 B), then 16 KiB + 128 B
+		// (src/Padding.c:24-41)
+		p->n = 2;
+		p->mod[0] = 16 * 1024 * 1024; p->off[0] = 2 * 1024 * 1024 + 512;
+		p->mod[1] = 16 * 1024;        p->off[1] = 128;
+		return true;
+	}
+	if (!strncmp("no", s, 2) || !strncmp("off", s, 3)) return true;
+
+	int commas = 0;
+	for (const char * t = s; *t; ++t) commas += (*t == ',');
+	if (commas + 1 > 16) return fail("too many entries in padding string.");
+	int idx = 0, entries = 0;
+	bool haveMod = false, haveOff = false;
+	for (const char * t = s; *t;) {
+		if (*t == ',') {
+			if (!haveMod) return fail("modulus missing before next section in padding string.");
+			haveMod = haveOff = false;
+			++idx; ++t;
+		} else if (*t == '+') {
+			if (!haveMod) return fail("modulus missing before offset in padding string.");
+			if (haveOff) return fail("offset is only allowed to be specified once per section in padding string.");
+			++t;
+		} else if (*t >= '0' && *t <= '9') {
+			char * end = nullptr;
+			errno = 0;
+			long v = strtol(t, &end, 10);
+			if (errno != 0 || v > INT_MAX) return fail("over- or underflow in modulus/offset of padding string.");
+			if (!haveMod) { p->mod[idx] = (int)v; haveMod = true; ++entries; }
+			else {
+				if (haveOff) return fail("offset is only allowed to be specified once per section in padding string.");
+				p->off[idx] = (int)v; haveOff = true;
+			}
+			t = end;
+		} else {
+			return fail("padding string contains invalid characters.");
+		}
+	}
+	if (entries != commas + 1) return fail("did not find all padding entries.");
+	for (int i = 0; i < entries; ++i)
+		if (p->off[i] >= p->mod[i]) return fail("offset in padding entry %d is equal or larger than modulus.", i);
+	p->n = entries;
+	return true;
+}
+
+long pad_cells(long n, int cellBytes, const PadSpec * ps)
+{
+	PadSpec autoSpec;
+	if (ps == nullptr) { pad_parse("auto", &autoSpec); ps = &autoSpec; }
+	for (int i = 0; i < ps->n; ++i) {
+		long add;
+		if (ps->mod[i] == 0) {
+			add = (long)ps->off[i] * cellBytes;    // the reference scales the bare offset by the cell size
+		} else {
+			long m = ps->mod[i] / cellBytes;
+			if (m == 0) {
+				fail("modulus of %d byte in padding entry %d becomes zero for PDF size %d byte.",
+				     ps->mod[i], i, cellBytes);
+				return -1;
+			}
+			long o = ps->off[i] / cellBytes;
+			add = (o + m - n % m) % m;
+		}
+		n += add;
+	}
+	return n;
+}
+
+// ------------------------------------------------------------ kernel args
+static bool arg_int(const Parameters * p, int & i, int * out, const char * name, int lo, int hi)
+{
+	if (i + 1 >= p->nKernelArgs) return fail("argument %s requires a parameter.", name);
+	char * end = nullptr;
+	long v = strtol(p->KernelArgs[++i], &end, 0);
+	if (end == p->KernelArgs[i] || *end != '\0') return fail("argument %s expects an integer.", name);
+	if (v < lo || v > hi) return fail("argument %s must be in [%d, %d].", name, lo, hi);
+	*out = (int)v;
+	return true;
+}
+
+bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
+{
+	*a = KernelArgs();
+	if (p == nullptr) return true;
+	for (int i = 0; i < p->nKernelArgs; ++i) {
+		const char * s = p->KernelArgs[i];
+		auto is = [&](const char * x) { return !strcmp(s, x); };
+		if (is("-blk") || is("--blk")) {
+			int v;
+			if (!arg_int(p, i, &v, "-blk", 0, INT_MAX)) return false;
+			a->blk[0] = a->blk[1] = a->blk[2] = v;
+		} else if (is("-blk-x") || is("--blk-x")) {
+			if (!arg_int(p, i, &a->blk[0], "-blk-x", 0, INT_MAX)) return false;
+		} else if (is("-blk-y") || is("--blk-y")) {
+			if (!arg_int(p, i, &a->blk[1], "-blk-y", 0, INT_MAX)) return false;
+		} else if (is("-blk-z") || is("--blk-z")) {
+			if (!arg_int(p, i, &a->blk[2], "-blk-z", 0, INT_MAX)) return false;
+		} else if (listKernel && (is("-pad") || is("--pad"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument %s requires a parameter.", s);
+			if (!pad_parse(p->KernelArgs[++i], &a->pad)) return false;
+			a->havePad = true;
+		} else if (is("-force") || is("--force")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -force requires on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "on")) a->forceOn = 1;
+			else if (!strcmp(v, "off")) a->forceOn = 0;
+			else return fail("argument -force requires on or off.");
+		} else if (is("-lambda") || is("--lambda")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -lambda requires a value.");
+			char * end = nullptr;
+			double v = strtod(p->KernelArgs[++i], &end);
+			if (end == p->KernelArgs[i] || *end != '\0' || !(v > 0.0)) return fail("argument -lambda expects a positive number.");
+			a->magic = v;
+		} else if (is("-bs") || is("--bs")) {
+			if (!arg_int(p, i, &a->blockSize, "-bs", 64, 1024)) return false;
+			if (a->blockSize % 64) return fail("argument -bs must be a multiple of 64.");
+		} else if (is("-h") || is("-help") || is("--help")) {
+			printf("Kernel parameters:\n");
+			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
+			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
+			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>]\n");
+			return fail("kernel help requested.");
+		} else {
+			return fail("unknown kernel parameter: %s.", s);
+		}
+	}
+	return true;
+}
+
+// ---------------------------------------------------------------- physics
+TrtConsts trt_consts(double omega, double magic)
+{
+	TrtConsts k;
+	k.oe = omega;
+	k.oo = 1.0 / (0.5 + magic / (1.0 / omega - 0.5));
+	k.w0 = 1.0 / 3.0;
+	k.w1 = 1.0 / 18.0;
+	k.w2 = 1.0 / 36.0;
+	k.w1x3 = k.w1 * 3.0;
+	k.w1nh = k.w1 * 9.0 / 2.0;
+	k.w2x3 = k.w2 * 3.0;
+	k.w2nh = k.w2 * 9.0 / 2.0;
+	return k;
+}
+
+ForceTerms force_terms(double F)
+{
+	static const double w[19] = {
+		1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+		1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+		1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 3.0 };
+	ForceTerms t;
+	for (int d = 0; d < 19; ++d) t.f[d] = 3.0 * w[d] * kCx[d] * F;
+	return t;
+}
+
+// ----------------------------------------------------------- common state
+bool common_init(Common * c, const Parameters * params, bool listKernel)
+{
+	if (!parse_kernel_args(params, &c->args, listKernel)) return false;
+	int n = 0;
+	const hipError_t e = hipGetDeviceCount(&n);
+	if (e == hipErrorNoDevice || (e == hipSuccess && n < 1))
+		return fail("no HIP device found; the list/aa kernels of this library run on the GPU only.");
+	if (!hip_ok(e, "hipGetDeviceCount")) return false;
+	if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) return false;
+	if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+	if (!hip_ok(hipEventCreate(&c->evLoop[0]), "hipEventCreate")) return false;
+	if (!hip_ok(hipEventCreate(&c->evLoop[1]), "hipEventCreate")) return false;
+	return true;
+}
+
+void common_free(Common * c)
+{
+	for (int i = 0; i < c->nEv; ++i) (void)hipEventDestroy(c->ev[i]);
+	free(c->ev);
+	c->ev = nullptr;
+	c->nEv = 0;
+	for (auto & e : c->evLoop) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+	if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
+}
+
+bool common_events(Common * c, int launches)
+{
+	int need = 2 * launches;
+	if (need <= c->nEv) return true;
+	hipEvent_t * ev = (hipEvent_t *)realloc(c->ev, sizeof(hipEvent_t) * (size_t)need);
+	if (!ev) return fail("allocating %d events failed.", need);
+	c->ev = ev;
+	for (int i = c->nEv; i < need; ++i) {
+		if (!hip_ok(hipEventCreate(&c->ev[i]), "hipEventCreate")) { c->nEv = i; return false; }
+	}
+	c->nEv = need;
+	return true;
+}
+
+void common_collect(Common * c, int launches, bool perLaunch)
+{
+	float ms = 0.0f;
+	(void)hipEventElapsedTime(&ms, c->evLoop[0], c->evLoop[1]);
+	c->t.loopMs = ms;
+	c->t.steps = launches;
+	c->t.evenMsAvg = c->t.oddMsAvg = 0.0;
+	c->t.evenMsMin = c->t.oddMsMin = 0.0;
+	if (!perLaunch || launches == 0) return;
+	double se = 0, so = 0, me = 1e30, mo = 1e30;
+	int ne = 0, no = 0;
+	for (int i = 0; i < launches; ++i) {
+		float m = 0.0f;
+		(void)hipEventElapsedTime(&m, c->ev[2 * i], c->ev[2 * i + 1]);
+		if (i % 2 == 0) { se += m; ++ne; if (m < me) me = m; }
+		else            { so += m; ++no; if (m < mo) mo = m; }
+	}
+	c->t.evenMsAvg = ne ? se / ne : 0.0;
+	c->t.oddMsAvg = no ? so / no : 0.0;
+	c->t.evenMsMin = ne ? me : 0.0;
+	c->t.oddMsMin = no ? mo : 0.0;
+}
+
+void * host_alloc(size_t bytes)
+{
+	void * p = nullptr;
+	if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+	return p;
+}
+
+void host_free(void * p)
+{
+	if (p) (void)hipHostFree(p);
+}
+
+}  // namespace lbm
+
+// -------------------------------------------------------------- C exports
+extern "C" {
+
+void LbmHipSetErrorMode(int mode)
+{
+	lbm::g_errorMode = (mode == LBM_HIP_ERRORS_RETURN) ? LBM_HIP_ERRORS_RETURN : LBM_HIP_ERRORS_EXIT;
+}
+
+const char * LbmHipLastError(void) { return lbm::g_lastError; }
+
+void LbmHipClearError(void) { lbm::g_lastError[0] = '\0'; }
+
+int LbmHipDeviceInfo(char * name, int nameLen, int * cus, double * memClockGHz)
+{
+	// a query: reports -1 without the error path when no device is present
+	int dev = 0, n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return -1;
+	if (hipGetDevice(&dev) != hipSuccess) return -1;
+	hipDeviceProp_t p;
+	if (hipGetDeviceProperties(&p, dev) != hipSuccess) return -1;
+	if (name && nameLen > 0) snprintf(name, (size_t)nameLen, "%s (%s)", p.name, p.gcnArchName);
+	if (cus) *cus = p.multiProcessorCount;
+	if (memClockGHz) *memClockGHz = p.memoryClockRate / 1.0e6;
+	return 0;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------- kernel table
+// Laid out like the reference's g_kernels[] (src/KernelFunctions.h:56-188);
+// a driver looks kernels up by name with strcasecmp (src/Main.c:537-553).
+extern "C" {
+KernelFunctions g_lbmHipKernels[] = {
+	{ "list-aa-pv-soa", D3Q19ListAaPvInit_PushSoAHip, D3Q19ListAaPvDeinit_PushSoAHip },
+	{ "aa-soa",         D3Q19AaInit_AaSoAHip,         D3Q19AaDeinit_AaSoAHip },
+};
+const int g_lbmHipKernelCount = (int)(sizeof(g_lbmHipKernels) / sizeof(g_lbmHipKernels[0]));
+}
+
+namespace lbm {
+Common * common_of(KernelData * kd)
+{
+	if (!kd) return nullptr;
+	Common * c = list_common(kd);
+	return c ? c : aa_common(kd);
+}
+}  // namespace lbm
+
+extern "C" {
+
+int LbmHipGetTimings(const KernelData * kd, LbmHipTimings * out)
+{
+	lbm::Common * c = lbm::common_of(const_cast<KernelData *>(kd));
+	if (!c || !out) { lbm::fail("LbmHipGetTimings: not a kernel of this library."); return -1; }
+	*out = c->t;
+	return 0;
+}
+
+int LbmHipSetLaunchEvents(KernelData * kd, int on)
+{
+	lbm::Common * c = lbm::common_of(kd);
+	if (!c) { lbm::fail("LbmHipSetLaunchEvents: not a kernel of this library."); return -1; }
+	c->launchEvents = on != 0;
+	return 0;
+}
+
+int LbmHipSetShadowDownload(KernelData * kd, int on)
+{
+	lbm::Common * c = lbm::common_of(kd);
+	if (!c) { lbm::fail("LbmHipSetShadowDownload: not a kernel of this library."); return -1; }
+	c->shadowDownload = on != 0;
+	return 0;
+}
+
+int LbmHipSyncShadow(KernelData * kd)
+{
+	lbm::Common * c = lbm::common_of(kd);
+	if (!c || !c->download) { lbm::fail("LbmHipSyncShadow: not a kernel of this library."); return -1; }
+	return c->download(kd) ? 0 : -1;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int LbmHipDumpNodes(KernelData * kd, const LatticeDesc * ld, PdfT * out)
+{
+	if (!lbm::common_of(kd) || !ld || !out) { lbm::fail("LbmHipDumpNodes: invalid arguments."); return -1; }
+	size_t k = 0;
+	for (int z = 0; z < ld->Dims[2]; ++z)
+	for (int y = 0; y < ld->Dims[1]; ++y)
+	for (int x = 0; x < ld->Dims[0]; ++x) {
+		if (ld->Lattice[((size_t)z * ld->Dims[1] + y) * ld->Dims[0] + x] == LBM_CELL_OBSTACLE) continue;
+		kd->GetNode(kd, x, y, z, out + 19 * k);
+		++k;
+	}
+	return 0;
+}
+
+int LbmHipLoadNodes(KernelData * kd, const LatticeDesc * ld, const PdfT * in)
+{
+	if (!lbm::common_of(kd) || !ld || !in) { lbm::fail("LbmHipLoadNodes: invalid arguments."); return -1; }
+	size_t k = 0;
+	for (int z = 0; z < ld->Dims[2]; ++z)
+	for (int y = 0; y < ld->Dims[1]; ++y)
+	for (int x = 0; x < ld->Dims[0]; ++x) {
+		if (ld->Lattice[((size_t)z * ld->Dims[1] + y) * ld->Dims[0] + x] == LBM_CELL_OBSTACLE) continue;
+		kd->SetNode(kd, x, y, z, const_cast<PdfT *>(in + 19 * k));
+		++k;
+	}
+	return 0;
+}
+
+}  // extern "C"

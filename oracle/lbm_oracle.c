@@ -357,8 +357,10 @@ static void list_force(orc_list_t * L, double F)
 	double ft[ORC_NDIR];
 	for (int d = 0; d < ORC_NDIR; ++d) ft[d] = orc_force_term(d, F);
 	const int X = L->dims[0], Y = L->dims[1], Z = L->dims[2];
-	double v[ORC_NDIR];
+	/* every slot belongs to exactly one node view: planes are independent */
+	#pragma omp parallel for schedule(static)
 	for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+		double v[ORC_NDIR];
 		if (L->mask[lidx(L->dims, x, y, z)] == 0) continue;
 		orc_list_get_node(L, x, y, z, v);
 		for (int d = 0; d < ORC_NDIR; ++d) v[d] = v[d] + ft[d];
@@ -537,9 +539,11 @@ void orc_aa_set_node(orc_aa_t * A, int x, int y, int z, const double in[ORC_NDIR
 
 static void aa_force(orc_aa_t * A, double F)
 {
-	double ft[ORC_NDIR], v[ORC_NDIR];
+	double ft[ORC_NDIR];
 	for (int d = 0; d < ORC_NDIR; ++d) ft[d] = orc_force_term(d, F);
+	#pragma omp parallel for schedule(static)
 	for (int z = 0; z < A->dims[2]; ++z) for (int y = 0; y < A->dims[1]; ++y) for (int x = 0; x < A->dims[0]; ++x) {
+		double v[ORC_NDIR];
 		if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
 		orc_aa_get_node(A, x, y, z, v);
 		for (int d = 0; d < ORC_NDIR; ++d) v[d] = v[d] + ft[d];

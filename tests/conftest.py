@@ -1,0 +1,51 @@
+"""Shared fixtures.  GPU tests are marked ``gpu``; everything else runs on CPU."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "lbm-benchmark-kernels_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+for p in (PKG, ORACLE):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture(scope="session")
+def lbm():
+    import lbm_amd
+    return lbm_amd
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from lbm_checkers import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def reference():
+    from lbm_checkers import Reference
+    if not Reference.available("verif"):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    return Reference("verif")
+
+
+def golden(name: str):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name} not generated")
+    return np.load(path, allow_pickle=False)
+
+
+def tiny_cases():
+    return sorted(f[5:-4] for f in os.listdir(GOLDEN) if f.startswith("tiny_") and f.endswith(".npz"))

@@ -1,0 +1,137 @@
+"""The drop-in boundary without a GPU: libraries load, export every symbol
+the headers declare, struct layouts match the reference's, and the entry
+points fail loudly (no CPU fallback) when no device is present."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import PKG, ROOT
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include HDR
+#define OFS(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
+int main(void) {
+  printf("LatticeDesc %zu\n", sizeof(LatticeDesc));
+  OFS(LatticeDesc, Lattice) OFS(LatticeDesc, nCells) OFS(LatticeDesc, PeriodicZ) OFS(LatticeDesc, Name)
+  printf("CaseData %zu\n", sizeof(CaseData));
+  OFS(CaseData, XForce) OFS(CaseData, MaxIterations) OFS(CaseData, StatisticsModulus)
+  printf("KernelData %zu\n", sizeof(KernelData));
+  OFS(KernelData, PdfsActive) OFS(KernelData, Dims) OFS(KernelData, ObstIndices) OFS(KernelData, nBounceBackPdfs)
+  OFS(KernelData, LoopBalance) OFS(KernelData, Duration) OFS(KernelData, BoundaryConditionsGetPdf)
+  OFS(KernelData, GetNode) OFS(KernelData, SetNode) OFS(KernelData, Kernel)
+  printf("Parameters %zu\n", sizeof(Parameters));
+  OFS(Parameters, KernelArgs)
+  printf("KernelFunctions %zu\n", sizeof(KernelFunctions));
+  OFS(KernelFunctions, Init) OFS(KernelFunctions, Deinit)
+  return 0;
+}
+"""
+
+
+def _probe(tmp_path, header, incs, defines=()):
+    src = tmp_path / "probe.c"
+    src.write_text(PROBE.replace("HDR", f'"{header}"'))
+    exe = tmp_path / "probe"
+    # the reference header also defines its kernel table; nothing in it is called
+    cmd = ["gcc", "-std=c99", *[f"-I{i}" for i in incs], *defines, str(src), "-o", str(exe),
+           "-Wl,--unresolved-symbols=ignore-all"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+
+
+def test_layout_matches_ctypes_mirror(tmp_path, lbm):
+    got = _probe(tmp_path, "lbm_kernel_abi.h", [os.path.join(ROOT, "include")])
+    for name, T in [("LatticeDesc", lbm.LatticeDesc), ("CaseData", lbm.CaseData), ("KernelData", lbm.KernelData),
+                    ("Parameters", lbm.Parameters), ("KernelFunctions", lbm.KernelFunctions)]:
+        assert int(got[name]) == C.sizeof(T), name
+        for key, v in got.items():
+            if key.startswith(name + "."):
+                assert int(v) == getattr(T, key.split(".")[1]).offset, key
+
+
+def test_layout_matches_reference_headers(tmp_path):
+    ref = "/root/reference/src"
+    if not os.path.exists(os.path.join(ref, "Kernel.h")):
+        pytest.skip("reference headers not present")
+    mine = _probe(tmp_path, "lbm_kernel_abi.h", [os.path.join(ROOT, "include")])
+    (tmp_path / "r").mkdir()
+    theirs = _probe(tmp_path / "r", "KernelFunctions.h", [ref],
+                    ["-DPRECISION_DP", "-DDATA_LAYOUT_SOA", "-DPROP_MODEL_PUSH", "-D_POSIX_C_SOURCE=200112L",
+                     "-DVECTOR_AVX"])
+    assert mine == theirs
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    funcs = set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", txt, flags=re.M))
+    data = set(re.findall(r"^extern\s+[\w\s\*]+?\b(\w+)\s*(?:\[[^\]]*\])?;", txt, flags=re.M))
+    return {f for f in funcs | data if not f.startswith(("typedef", "enum"))} - {"void", "int", "const"}
+
+
+def _exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(PKG, lib)], check=True,
+                         capture_output=True, text=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+@pytest.mark.parametrize("header,lib", [("lbm_hip.h", "liblbmhip.so"), ("lbm_host.h", "liblbmhost.so")])
+def test_exports_every_declared_symbol(header, lib):
+    declared = _declared(header)
+    assert declared, header
+    missing = declared - _exported(lib)
+    assert not missing, f"{lib} lacks {sorted(missing)}"
+
+
+def test_symbol_lists_in_mirror(lbm):
+    assert set(lbm.HIP_SYMBOLS) <= _exported("liblbmhip.so")
+    assert set(lbm.HOST_SYMBOLS) <= _exported("liblbmhost.so")
+    # the HIP library must not clash with the reference's Kernel.o/Geometry.o
+    # when both are linked into the reference driver
+    clash = {"KernelDensity", "KernelVerifiy", "GeoCreateByStr", "PadCells", "D3Q19_X", "MemAlloc"}
+    assert not (clash & _exported("liblbmhip.so"))
+
+
+def test_kernel_table(lbm):
+    assert list(lbm.kernel_table()) == ["list-aa-pv-soa", "aa-soa"]
+
+
+def _no_gpu(lbm):
+    return lbm.device_info() is None
+
+
+def test_init_fails_loudly_without_device(lbm):
+    if not _no_gpu(lbm):
+        pytest.skip("a GPU is present")
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    for name in ("list-aa-pv-soa", "aa-soa"):
+        with pytest.raises(RuntimeError, match="no HIP device"):
+            lbm.Kernel(name, geo)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-foo"], "unknown kernel parameter"),
+    (["-blk"], "requires a parameter"),
+    (["-blk", "-3"], "must be in"),
+    (["-pad", "64+64"], "equal or larger than modulus"),
+    (["-pad", "x"], "invalid characters"),
+    (["-force", "maybe"], "on or off"),
+    (["-bs", "100"], "multiple of 64"),
+    (["-lambda", "-1"], "positive"),
+])
+def test_kernel_argument_errors(lbm, args, msg):
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with pytest.raises(RuntimeError, match=msg):
+        lbm.Kernel("list-aa-pv-soa", geo, args)
+
+
+def test_aa_rejects_list_only_argument(lbm):
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with pytest.raises(RuntimeError, match="unknown kernel parameter"):
+        lbm.Kernel("aa-soa", geo, ["-pad", "auto"])

@@ -1,0 +1,38 @@
+"""The reference's own test protocol (src/test-verification.sh:52-112): run
+every kernel with -verify and require exit code 0, i.e. the analytic-profile
+deviation <= 0.1 -- here through the lbmbenchk-hip driver on the GPU."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import PKG
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(PKG, "lbmbenchk-hip")
+
+
+@pytest.mark.parametrize("kernel", ["list-aa-pv-soa", "aa-soa"])
+@pytest.mark.parametrize("kargs", [[], ["--", "-blk", "7"]])
+def test_verify(tmp_path, kernel, kargs):
+    if kernel == "aa-soa" and kargs:
+        kargs = []
+    print(f"driver -verify -kernel {kernel} {' '.join(kargs)}", flush=True)
+    r = subprocess.run([DRIVER, "-verify", "-kernel", kernel, *kargs], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "# VERIFICATION: deviation from analytical solution: 4.561374e-02" in r.stdout
+    assert "VERIFICATION SUCCEEDED" in r.stdout
+    p_line = [line for line in r.stdout.splitlines() if line.startswith("P:")]
+    assert len(p_line) == 1 and "MFLUP/s" in p_line[0]
+    assert (tmp_path / "flow-profile.dat").exists()
+    assert (tmp_path / "density-ux.dat").exists()
+
+
+def test_driver_porous_run(tmp_path):
+    r = subprocess.run([DRIVER, "-dims", "64x64x64", "-geometry", "porous-4-0.8-1", "-iterations", "100",
+                        "-omega", "1.2", "-x-force", "1e-6", "-kernel", "list-aa-pv-soa"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "porous medium" in r.stdout
