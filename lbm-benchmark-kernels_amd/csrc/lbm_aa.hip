@@ -146,7 +146,11 @@ __global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * 
 // (:435-487) -- or to the cell's own slot inv(d) when the target is solid.
 // FORCE_STORE adds this step's force increment to every stored value that
 // lands in a cell's node view (not to ghost slots, which no node view
-// covers after an odd step).
+// covers after an odd step).  A direction whose pull neighbour lies beyond
+// a non-periodic face leaves the cell's own slot d unwritten in this step
+// (its writer would be the ghost cell); that slot is still part of the
+// node view the reference forces after the step, so it gets the increment
+// here too.
 template <bool FORCE_LOAD, bool FORCE_STORE>
 __global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
                                                 TrtConsts k, ForceTerms ft)
@@ -159,6 +163,7 @@ __global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * _
 	const uint64_t self = aa_slot(g, 0, x, y, z);
 
 	uint64_t src[18], dst[18];
+	uint32_t ghostPull = 0;
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
 		const int id = inv_dir(d);
@@ -172,6 +177,7 @@ __global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * _
 			int qx = x - cx(d), qy = y - cy(d), qz = z - cz(d);
 			aa_resolve(g, qx, qy, qz);
 			src[d] = aa_slot(g, id, qx, qy, qz);
+			if (qx < 0 || qx >= g.X || qy < 0 || qy >= g.Y || qz < 0 || qz >= g.Z) ghostPull |= 1u << d;
 		}
 		// store of direction d
 		if (w & (1u << d)) {
@@ -211,29 +217,40 @@ __global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * _
 		pdf[dst[d]] = v;
 	}
 	pdf[cSlot] = FORCE_STORE ? f[dC] + ft.f[dC] : f[dC];
+	if (FORCE_STORE && ghostPull) {
+#pragma unroll
+		for (int d = 0; d < 18; ++d) {
+			if (ghostPull & (1u << d)) {
+				const uint64_t own = self + (uint64_t)d * g.plane;
+				pdf[own] = pdf[own] + ft.f[d];
+			}
+		}
+	}
 }
 
 // ----------------------------------------------------------- node access
 // Before the first and after every Kernel() call (an even number of steps)
 // the node view is local: direction d of a cell in its own slot d
 // (src/BenchKernelD3Q19AaCommon.c:153-158).
-static bool aa_local_layout(const AaData * A)
+static bool aa_local_layout(const AaData * A, int x, int y, int z)
 {
-	if (A->c.iteration % 2 != 0) return true;
-	return fail("aa-soa: node access after an even step is not supported by this build.");
+	if (A->c.iteration % 2 == 0) return fail("aa-soa: node access after an even step is not supported by this build.");
+	if (x < 0 || y < 0 || z < 0 || x >= A->g.X || y >= A->g.Y || z >= A->g.Z)
+		return fail("aa-soa: node (%d, %d, %d) outside the lattice.", x, y, z);
+	return true;
 }
 
 static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A)) return;
+	if (!aa_local_layout(A, x, y, z)) return;
 	for (int d = 0; d < 19; ++d) pdfs[d] = A->hPdf[aa_slot(A->g, d, x, y, z)];
 }
 
 static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A)) return;
+	if (!aa_local_layout(A, x, y, z)) return;
 	for (int d = 0; d < 19; ++d) A->hPdf[aa_slot(A->g, d, x, y, z)] = pdfs[d];
 	A->c.shadowDirty = 1;
 }
@@ -241,14 +258,14 @@ static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 static void aa_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A)) return;
+	if (!aa_local_layout(A, x, y, z)) return;
 	*pdf = A->hPdf[aa_slot(A->g, dir, x, y, z)];
 }
 
 static void aa_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A)) return;
+	if (!aa_local_layout(A, x, y, z)) return;
 	A->hPdf[aa_slot(A->g, dir, x, y, z)] = pdf;
 	A->c.shadowDirty = 1;
 }
@@ -290,6 +307,7 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
 
+	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
 	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
@@ -309,7 +327,8 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
 	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
 
-	common_collect(&c, steps, perLaunch);
+	common_collect(&c, A->nFluid > 0 ? steps : 0, perLaunch);
+	c.t.steps = steps;
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
@@ -379,6 +398,7 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		if (!hip_ok(hipMemcpyAsync(dMask, ld->Lattice, sizeof(int) * nTotal, hipMemcpyHostToDevice, s), "mask upload")) break;
 		if (!hip_ok(hipMemsetAsync(dCount, 0, sizeof(uint32_t), s), "hipMemset")) break;
 		if (!hip_ok(hipMemsetAsync(A->dPdf, 0, sizeof(double) * A->nPdf, s), "hipMemset")) break;
+		(void)hipGetLastError();
 		k_aa_info<<<(unsigned)((nTotal + 255) / 256), 256, 0, s>>>(dMask, g, A->dInfo, dCount);
 		if (!hip_ok(hipGetLastError(), "cell info kernel")) break;
 		if (!hip_ok(hipMemcpyAsync(&hCount, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count download")) break;

@@ -296,19 +296,32 @@ static bool list_local_layout(const ListData * L)
 	return fail("list-aa-pv-soa: node access after an even step is not supported by this build.");
 }
 
+// Node id of a fluid cell, or false (with the error recorded) for a cell
+// outside the box or a solid cell -- the reference asserts both only in
+// debug builds (src/BenchKernelD3Q19ListAaPvCommon.c:133-134).
+static bool list_node(const ListData * L, int x, int y, int z, uint32_t * n)
+{
+	if (!list_local_layout(L)) return false;
+	if (x < 0 || y < 0 || z < 0 || x >= L->dims[0] || y >= L->dims[1] || z >= L->dims[2])
+		return fail("list-aa-pv-soa: node (%d, %d, %d) outside the lattice.", x, y, z);
+	*n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	if (*n == 0xffffffffu) return fail("list-aa-pv-soa: node (%d, %d, %d) is solid.", x, y, z);
+	return true;
+}
+
 static void list_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	ListData * L = as_list(kd);
-	if (!list_local_layout(L)) return;
-	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	uint32_t n;
+	if (!list_node(L, x, y, z, &n)) return;
 	for (int d = 0; d < 19; ++d) pdfs[d] = L->hPdf[(size_t)d * L->nCells + n];
 }
 
 static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	ListData * L = as_list(kd);
-	if (!list_local_layout(L)) return;
-	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	uint32_t n;
+	if (!list_node(L, x, y, z, &n)) return;
 	for (int d = 0; d < 19; ++d) L->hPdf[(size_t)d * L->nCells + n] = pdfs[d];
 	L->c.shadowDirty = 1;
 }
@@ -316,16 +329,16 @@ static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 static void list_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
 {
 	ListData * L = as_list(kd);
-	if (!list_local_layout(L)) return;
-	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	uint32_t n;
+	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19) return;
 	*pdf = L->hPdf[(size_t)dir * L->nCells + n];
 }
 
 static void list_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 {
 	ListData * L = as_list(kd);
-	if (!list_local_layout(L)) return;
-	const uint32_t n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
+	uint32_t n;
+	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19) return;
 	L->hPdf[(size_t)dir * L->nCells + n] = pdf;
 	L->c.shadowDirty = 1;
 }
@@ -370,6 +383,7 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const dim3 grid((unsigned)((L->nFluid + bs - 1) / bs)), block((unsigned)bs);
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells, st = (uint32_t)L->adjStride;
 
+	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
 	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
@@ -389,7 +403,8 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
 	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
 
-	common_collect(&c, steps, perLaunch);
+	common_collect(&c, L->nFluid > 0 ? steps : 0, perLaunch);
+	c.t.steps = steps;
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
@@ -441,6 +456,7 @@ static bool list_build(ListData * L, const LatticeDesc * ld)
 		// count first: coords and adjacency are sized by the count, which
 		// must agree with LatticeDesc.nFluid before anything is written
 		NumGeom g = {X, Y, Z, L->blk[0], L->blk[1], L->blk[2]};
+		(void)hipGetLastError();
 		k_num_count<<<nBlocks, kNumBlock, 0, s>>>(dMask, g, N, dSums);
 		k_scan_once<<<1, 1024, 0, s>>>(dSums, nBlocks, dSums + nBlocks);
 		if (!hip_ok(hipGetLastError(), "numbering kernels")) break;

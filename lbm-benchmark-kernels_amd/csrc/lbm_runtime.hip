@@ -247,7 +247,7 @@ void common_collect(Common * c, int launches, bool perLaunch)
 	c->t.steps = launches;
 	c->t.evenMsAvg = c->t.oddMsAvg = 0.0;
 	c->t.evenMsMin = c->t.oddMsMin = 0.0;
-	if (!perLaunch || launches == 0) return;
+	if (!perLaunch || launches == 0) { (void)hipGetLastError(); return; }
 	double se = 0, so = 0, me = 1e30, mo = 1e30;
 	int ne = 0, no = 0;
 	for (int i = 0; i < launches; ++i) {
@@ -260,6 +260,7 @@ void common_collect(Common * c, int launches, bool perLaunch)
 	c->t.oddMsAvg = no ? so / no : 0.0;
 	c->t.evenMsMin = ne ? me : 0.0;
 	c->t.oddMsMin = no ? mo : 0.0;
+	(void)hipGetLastError();
 }
 
 void * host_alloc(size_t bytes)

@@ -231,8 +231,13 @@ def test_setnode_roundtrip_and_upload(lbm):
             v = rng.random((geo.n_fluid, 19))
             k.load(v)
             assert np.array_equal(k.dump(), v)
-            k.set_node(1, 2, 3, np.arange(19.0))
-            assert np.array_equal(k.get_node(1, 2, 3), np.arange(19.0))
-            # zero steps round up to none: data goes to the device and back unchanged
+            assert geo.mask[2, 2, 1] != 0
+            k.set_node(1, 2, 2, np.arange(19.0))
+            assert np.array_equal(k.get_node(1, 2, 2), np.arange(19.0))
+            # zero steps: data goes to the device and back unchanged
             k.run(lbm.case_data(iterations=0))
-            assert np.array_equal(k.get_node(1, 2, 3), np.arange(19.0))
+            assert np.array_equal(k.get_node(1, 2, 2), np.arange(19.0))
+    with lbm.Kernel("list-aa-pv-soa", geo) as k:
+        # (1, 2, 3) lies in the solid z = 3 wall: reported, not dereferenced
+        k.set_node(1, 2, 3, np.arange(19.0))
+        assert "is solid" in lbm.hip_lib().LbmHipLastError().decode()
