@@ -19,7 +19,10 @@
  *                     reference's VERIFICATION build does (default on)
  *   -lambda <v>       TRT magic parameter (default 1/12, the value the
  *                     reference hard-codes, src/BenchKernelD3Q19ListAaPv.c:177)
- *   -bs <n>           threads per block of the step kernels (64..1024, step 64)
+ *   -bs <n>           threads per block of the step kernels (64..512, step 64)
+ *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
+ *                     (default) or in full (18 entries per node)
+ *   -nt on|off        non-temporal loads/stores in the even step (default on)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H
@@ -98,7 +101,9 @@ typedef struct {
 	int dims[3];
 	int periodic[3];
 	double bytesPerFlupEven;
-	double bytesPerFlupOdd;
+	double bytesPerFlupOdd;    /* algorithmic bytes per node of the odd step as configured */
+	int nRunsWave;             /* runs when cut at every 64-node wavefront (run-compressed odd step) */
+	int adjRle;                /* 1: odd step reads the run-compressed adjacency (-adj rle) */
 } LbmHipListInfo;
 int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out);
 

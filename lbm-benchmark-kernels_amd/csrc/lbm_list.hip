@@ -41,7 +41,13 @@ struct ListData {
 	int nFluid = 0;
 	int nCells = 0;
 	int adjStride = 0;
-	int nRuns = 0;
+	int nRuns = 0;                // runs of consecutive adjacency (no forced breaks)
+	int nRunsWave = 0;            // runs with a forced break at every 64-node wavefront
+	int nWaves = 0;
+	int runStride = 0;
+	unsigned long long * dRunMask = nullptr;   // per wavefront: bit l = node 64w+l starts a run
+	uint32_t * dRunBase = nullptr;             // per wavefront: index of its first run
+	uint32_t * dRunAdj = nullptr;              // [18][runStride]: adjacency of each run's first node
 	uint32_t * dGrid = nullptr;
 	uint32_t * dCoords = nullptr;
 	uint32_t * dAdj = nullptr;
@@ -225,11 +231,83 @@ __global__ __launch_bounds__(256) void k_run_breaks(const uint32_t * __restrict_
 	if (__lane_id() == 0 && b) atomicAdd(breaks, (uint32_t)__popcll(b));
 }
 
+// Run-compressed adjacency for the odd step.  Inside a run of nodes whose
+// 18 entries all grow by one from node to node -- the reference's
+// "consecutive nodes" (src/BenchKernelD3Q19ListAaPvCommon.c:326-469) -- only
+// the first node's entries are stored; the others add their offset in the
+// run.  Runs are cut at every 64-node wavefront, so a wavefront finds its
+// runs from one 64-bit start mask and one base index.
+__global__ __launch_bounds__(256) void k_wave_runs(const uint32_t * __restrict__ adj, uint32_t nFluid, uint32_t stride,
+                                                   uint32_t nWaves, unsigned long long * __restrict__ runMask,
+                                                   uint32_t * __restrict__ runCount)
+{
+	const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+	int start = 0;
+	if (n < nFluid) {
+		start = (n & 63u) == 0;
+		if (!start)
+			for (int d = 0; d < 18; ++d)
+				if (adj[(size_t)d * stride + n] != adj[(size_t)d * stride + n - 1] + 1u) { start = 1; break; }
+	}
+	const unsigned long long b = __ballot(start);
+	const uint32_t w = n >> 6;
+	if (__lane_id() == 0 && w < nWaves) {
+		runMask[w] = b;
+		runCount[w] = (uint32_t)__popcll(b);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_run_compact(const uint32_t * __restrict__ adj, uint32_t nFluid, uint32_t stride,
+                                                     const unsigned long long * __restrict__ runMask,
+                                                     const uint32_t * __restrict__ runBase, uint32_t runStride,
+                                                     uint32_t * __restrict__ runAdj)
+{
+	const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+	if (n >= nFluid) return;
+	const uint32_t w = n >> 6, lane = n & 63u;
+	const unsigned long long m = runMask[w];
+	if (!((m >> lane) & 1ull)) return;
+	const uint32_t r = runBase[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+	for (int d = 0; d < 18; ++d) runAdj[(size_t)d * runStride + r] = adj[(size_t)d * stride + n];
+}
+
+// Adjacency of node n for the odd step, from the full table or the runs.
+struct AdjFull {
+	const uint32_t * adj;
+	uint32_t stride;
+	__device__ __forceinline__ void fetch(uint32_t n, uint32_t (&a)[18]) const
+	{
+#pragma unroll
+		for (int d = 0; d < 18; ++d) a[d] = adj[(size_t)d * stride + n];
+	}
+};
+
+struct AdjRuns {
+	const unsigned long long * mask;
+	const uint32_t * base;
+	const uint32_t * runAdj;
+	uint32_t runStride;
+	__device__ __forceinline__ void fetch(uint32_t n, uint32_t (&a)[18]) const
+	{
+		const uint32_t w = n >> 6;
+		const int lane = (int)(n & 63u);
+		// run starts at or before this node; lane 0 of a wavefront always starts one
+		const unsigned long long upTo = mask[w] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+		const uint32_t r = base[w] + (uint32_t)__popcll(upTo) - 1u;
+		const uint32_t off = (uint32_t)(lane - (63 - __clzll(upTo)));
+#pragma unroll
+		for (int d = 0; d < 18; ++d) a[d] = runAdj[(size_t)d * runStride + r] + off;
+	}
+};
+
 // ---------------------------------------------------------------- steps
 // Even step: node n loads its 19 local values, collides and stores
 // direction d into its own slot inv(d).  FORCE_LOAD adds the body-force
 // increment of the preceding odd step to each loaded value.
-template <bool FORCE_LOAD>
+// NT streams the lattice with non-temporal loads and stores: every value is
+// touched once per step, so caching it only evicts useful lines.
+template <bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid, uint32_t nCells,
                                                    TrtConsts k, ForceTerms ft)
 {
@@ -238,12 +316,17 @@ __global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid
 	double f[19];
 #pragma unroll
 	for (int d = 0; d < 19; ++d) {
-		f[d] = pdf[(size_t)d * nCells + n];
+		double * p = &pdf[(size_t)d * nCells + n];
+		f[d] = NT ? __builtin_nontemporal_load(p) : *p;
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	trt_collide(f, k);
 #pragma unroll
-	for (int d = 0; d < 19; ++d) pdf[(size_t)inv_dir(d) * nCells + n] = f[d];
+	for (int d = 0; d < 19; ++d) {
+		double * p = &pdf[(size_t)inv_dir(d) * nCells + n];
+		if (NT) __builtin_nontemporal_store(f[d], p);
+		else *p = f[d];
+	}
 }
 
 // Odd step: direction d arrives from slot adj[n][inv d] (the neighbour's
@@ -253,16 +336,14 @@ __global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid
 // race free.  FORCE_STORE adds the force increment of this step to each
 // stored value: a value stored through a bounce-back entry is read back by
 // the same node as direction inv(d), any other by the neighbour as d.
-template <bool FORCE_LOAD, bool FORCE_STORE>
-__global__ __launch_bounds__(512) void k_list_odd(double * pdf, const uint32_t * __restrict__ adj,
-                                                  uint32_t nFluid, uint32_t nCells, uint32_t stride,
+template <bool FORCE_LOAD, bool FORCE_STORE, class Adj>
+__global__ __launch_bounds__(512) void k_list_odd(double * pdf, Adj adjacency, uint32_t nFluid, uint32_t nCells,
                                                   TrtConsts k, ForceTerms ft)
 {
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	uint32_t a[18];
-#pragma unroll
-	for (int d = 0; d < 18; ++d) a[d] = adj[(size_t)d * stride + n];
+	adjacency.fetch(n, a);
 	double f[19];
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
@@ -364,6 +445,15 @@ static bool list_download(ListData * L)
 	return hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize");
 }
 
+// Algorithmic bytes per fluid node of the odd step: 19 loads and 19 stores
+// of 8 B plus the adjacency -- 18 x 4 B per node in full, or per run 18 x
+// 4 B plus 12 B of mask and base per 64-node wavefront.
+static double list_odd_bytes(const ListData * L)
+{
+	if (!L->c.args.adjRle || L->nFluid == 0) return 2.0 * 19 * 8 + 18 * 4.0;
+	return 2.0 * 19 * 8 + (72.0 * L->nRunsWave + 12.0 * L->nWaves) / L->nFluid;
+}
+
 static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 {
 	(void)ld;
@@ -381,22 +471,39 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 
 	const int bs = c.args.blockSize;
 	const dim3 grid((unsigned)((L->nFluid + bs - 1) / bs)), block((unsigned)bs);
-	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells, st = (uint32_t)L->adjStride;
+	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
+	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
+	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
+	const bool rle = c.args.adjRle != 0, nt = c.args.nontemporal != 0;
+	auto even = [&](bool fl) {
+		if (fl && nt)  k_list_even<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else if (fl)   k_list_even<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else if (nt)   k_list_even<false, true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else           k_list_even<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+	};
+	auto odd = [&](bool fl, bool fs) {
+		if (rle) {
+			if (fs)      k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
+			else if (fl) k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
+			else         k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
+		} else {
+			if (fs)      k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
+			else if (fl) k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
+			else         k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
+		}
+	};
 
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
 	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
-		if (force && it > 0) k_list_even<true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else                 k_list_even<false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		even(force && it > 0);
 		if (perLaunch) {
 			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
 			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);
 		}
-		if (force && last) k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
-		else if (force)    k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
-		else               k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, L->dAdj, nF, nC, st, k, ft);
+		odd(force, force && last);
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
 	}
 	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
@@ -422,6 +529,9 @@ static void list_release(ListData * L)
 	(void)hipFree(L->dGrid);
 	(void)hipFree(L->dCoords);
 	(void)hipFree(L->dAdj);
+	(void)hipFree(L->dRunMask);
+	(void)hipFree(L->dRunBase);
+	(void)hipFree(L->dRunAdj);
 	(void)hipFree(L->dPdf);
 	host_free(L->hPdf);
 	free(L->hGrid);
@@ -429,11 +539,43 @@ static void list_release(ListData * L)
 	delete L;
 }
 
+
 static bool dev_alloc(void ** p, size_t bytes, const char * what)
 {
 	hipError_t e = hipMalloc(p, bytes ? bytes : 1);
 	if (e != hipSuccess) return fail("allocating %s failed: %zu bytes (%s).", what, bytes, hipGetErrorString(e));
 	return true;
+}
+
+// Run tables for the odd step: start masks per wavefront, a prefix sum of
+// the run counts, then the compacted adjacency of every run's first node.
+static bool list_build_runs(ListData * L, hipStream_t s)
+{
+	const uint32_t nF = (uint32_t)L->nFluid;
+	L->nWaves = (int)((nF + 63) / 64);
+	const unsigned nb = (nF + 255) / 256;
+	uint32_t * dCount = nullptr;
+	uint32_t total = 0;
+	bool ok = false;
+	do {
+		if (!dev_alloc((void **)&L->dRunMask, sizeof(unsigned long long) * (size_t)L->nWaves, "run masks")) break;
+		if (!dev_alloc((void **)&L->dRunBase, sizeof(uint32_t) * (size_t)L->nWaves, "run bases")) break;
+		if (!dev_alloc((void **)&dCount, sizeof(uint32_t), "run count")) break;
+		k_wave_runs<<<nb, 256, 0, s>>>(L->dAdj, nF, (uint32_t)L->adjStride, (uint32_t)L->nWaves, L->dRunMask, L->dRunBase);
+		k_scan_once<<<1, 1024, 0, s>>>(L->dRunBase, (uint32_t)L->nWaves, dCount);
+		if (!hip_ok(hipGetLastError(), "run kernels")) break;
+		if (!hip_ok(hipMemcpyAsync(&total, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "run count download")) break;
+		if (!hip_ok(hipStreamSynchronize(s), "run kernels")) break;
+		L->nRunsWave = (int)total;
+		L->runStride = (int)(((total + 255u) / 256u) * 256u);
+		if (!dev_alloc((void **)&L->dRunAdj, sizeof(uint32_t) * 18 * (size_t)L->runStride, "run adjacency")) break;
+		k_run_compact<<<nb, 256, 0, s>>>(L->dAdj, nF, (uint32_t)L->adjStride, L->dRunMask, L->dRunBase,
+		                                 (uint32_t)L->runStride, L->dRunAdj);
+		if (!hip_ok(hipGetLastError(), "run compaction")) break;
+		ok = true;
+	} while (0);
+	(void)hipFree(dCount);
+	return ok;
 }
 
 static bool list_build(ListData * L, const LatticeDesc * ld)
@@ -481,6 +623,7 @@ static bool list_build(ListData * L, const LatticeDesc * ld)
 			if (!hip_ok(hipGetLastError(), "adjacency kernels")) break;
 			if (!hip_ok(hipMemcpyAsync(&hTotal[1], dSums, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "run count download")) break;
 		}
+		if (L->nFluid > 0 && !list_build_runs(L, s)) break;
 		L->hGrid = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)N);
 		if (!L->hGrid) { fail("allocating host grid failed: %zu bytes.", sizeof(uint32_t) * (size_t)N); break; }
 		if (!hip_ok(hipMemcpyAsync(L->hGrid, L->dGrid, sizeof(uint32_t) * (size_t)N, hipMemcpyDeviceToHost, s), "grid download")) break;
@@ -552,8 +695,9 @@ void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Pa
 	memset(L->hPdf, 0, pdfBytes);
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "init")) { list_release(L); return; }
 
-	const double even = 2.0 * 19 * 8, odd = 2.0 * 19 * 8 + 18 * 4.0;
-	printf("# total fluid nodes: %d   consecutive blocks: %d\n", L->nFluid, L->nRuns);
+	const double even = 2.0 * 19 * 8, odd = list_odd_bytes(L);
+	printf("# total fluid nodes: %d   consecutive blocks: %d   runs cut at wavefronts: %d\n",
+	       L->nFluid, L->nRuns, L->nRunsWave);
 	printf("# loop balance:          %.2f B/FLUP  even: %.2f B/FLUP  odd %.2f B/FLUP\n", (even + odd) / 2, even, odd);
 
 	KernelData * kd = &L->kd;
@@ -589,7 +733,9 @@ int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out)
 	out->nRuns = L->nRuns;
 	for (int i = 0; i < 3; ++i) { out->dims[i] = L->dims[i]; out->periodic[i] = L->periodic[i]; }
 	out->bytesPerFlupEven = 2.0 * 19 * 8;
-	out->bytesPerFlupOdd = 2.0 * 19 * 8 + 18 * 4.0;
+	out->bytesPerFlupOdd = list_odd_bytes(L);
+	out->nRunsWave = L->nRunsWave;
+	out->adjRle = L->c.args.adjRle;
 	return 0;
 }
 

@@ -40,6 +40,8 @@ struct KernelArgs {
 	int forceOn = 1;
 	double magic = 1.0 / 12.0;
 	int blockSize = 256;
+	int adjRle = 1;        // list: odd step reads run-compressed adjacency
+	int nontemporal = 1;   // streaming (non-temporal) loads/stores in the even step
 };
 // Parses Parameters.KernelArgs.  listKernel enables -pad (the reference
 // accepts it only for the list kernel, src/BenchKernelD3Q19ListAaPvCommon.c:302-308).

@@ -156,14 +156,27 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			double v = strtod(p->KernelArgs[++i], &end);
 			if (end == p->KernelArgs[i] || *end != '\0' || !(v > 0.0)) return fail("argument -lambda expects a positive number.");
 			a->magic = v;
+		} else if (listKernel && (is("-adj") || is("--adj"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -adj requires rle or full.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "rle")) a->adjRle = 1;
+			else if (!strcmp(v, "full")) a->adjRle = 0;
+			else return fail("argument -adj requires rle or full.");
+		} else if (is("-nt") || is("--nt")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -nt requires on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "on")) a->nontemporal = 1;
+			else if (!strcmp(v, "off")) a->nontemporal = 0;
+			else return fail("argument -nt requires on or off.");
 		} else if (is("-bs") || is("--bs")) {
-			if (!arg_int(p, i, &a->blockSize, "-bs", 64, 1024)) return false;
+			if (!arg_int(p, i, &a->blockSize, "-bs", 64, 512)) return false;
 			if (a->blockSize % 64) return fail("argument -bs must be a multiple of 64.");
 		} else if (is("-h") || is("-help") || is("--help")) {
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>]\n");
+			if (listKernel) printf("  [-adj rle|full]\n");
+			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off]\n");
 			return fail("kernel help requested.");
 		} else {
 			return fail("unknown kernel parameter: %s.", s);

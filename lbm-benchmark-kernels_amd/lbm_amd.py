@@ -81,7 +81,8 @@ class LbmHipTimings(C.Structure):
 class LbmHipListInfo(C.Structure):
     _fields_ = [("nFluid", C.c_int), ("nCells", C.c_int), ("adjStride", C.c_int), ("nRuns", C.c_int),
                 ("dims", C.c_int * 3), ("periodic", C.c_int * 3),
-                ("bytesPerFlupEven", C.c_double), ("bytesPerFlupOdd", C.c_double)]
+                ("bytesPerFlupEven", C.c_double), ("bytesPerFlupOdd", C.c_double),
+                ("nRunsWave", C.c_int), ("adjRle", C.c_int)]
 
 
 class GeoPorous(C.Structure):

@@ -241,3 +241,26 @@ def test_setnode_roundtrip_and_upload(lbm):
         # (1, 2, 3) lies in the solid z = 3 wall: reported, not dereferenced
         k.set_node(1, 2, 3, np.arange(19.0))
         assert "is solid" in lbm.hip_lib().LbmHipLastError().decode()
+
+
+@pytest.mark.parametrize("geom,dims", [("porous-3-0.75-9", (64, 48, 40)), ("blocks-3", (30, 24, 27))])
+def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
+    # run-compressed and full adjacency, with and without non-temporal
+    # accesses, all give the oracle's result bit for bit
+    geo = lbm.Geometry.from_string(geom, dims)
+    m = geo.mask.reshape(-1)
+    want, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1)
+    for args in (["-adj", "rle"], ["-adj", "full"], ["-adj", "rle", "-nt", "off"], ["-adj", "full", "-nt", "off"],
+                 ["-adj", "rle", "-bs", "128"], ["-adj", "rle", "-bs", "64", "-blk", "5"]):
+        with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
+            info = k.list_info()
+            assert info.adjRle == (args[1] == "rle")
+            assert info.nRunsWave >= info.nRuns
+            k.set_initial_weights()
+            k.run(lbm.case_data(omega=1.25, xforce=3e-5, iterations=30))
+            got = k.dump()
+        if "-blk" in args:
+            w2, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1, blk=(5, 5, 5))
+            assert np.array_equal(got, w2), args
+        else:
+            assert np.array_equal(got, want), args
