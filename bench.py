@@ -14,10 +14,15 @@ around the step loop and around every launch.  value = nFluid * steps /
 time / 1e6 (src/Main.c:594).
 
 roofline: the odd-step kernel (the longer of the two).  Algorithmic bytes
-per fluid node: 19 loads + 19 stores of 8 B plus 18 adjacency entries of
-4 B = 376 B (even step: 304 B).  achieved = 376 B * nFluid / mean odd-launch
-time.  traffic: HBM bytes per odd launch from rocprofv3 FETCH_SIZE +
-WRITE_SIZE (profiles/list_odd_porous256_traffic.json), null if absent.
+per fluid node: 19 loads + 19 stores of 8 B = 304 B plus the adjacency it
+reads -- run-compressed by default: 72 B per run of consecutive adjacency
+plus 12 B per 64-node wavefront (the library reports the exact figure);
+376 B with -adj full.  Even step: 304 B.  achieved = bytes/node * nFluid /
+mean odd-launch time.  traffic: HBM bytes per odd launch from rocprofv3
+FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size
+(profiles/list_odd_porous256_traffic.json), null if absent.  The copy and
+update-19 bandwidths of the device are measured in the same run as
+yardsticks (the reference rates its kernels against update-19 bandwidth).
 
 cpu_baseline: the reference's own list-aa-pv-soa (benchmark build, compiled
 from its sources into oracle/_ref) on the same geometry, OpenMP on the host
@@ -37,7 +42,7 @@ sys.path.insert(0, os.path.join(ROOT, "lbm-benchmark-kernels_amd"))
 GEOMETRY = "porous-6-0.8-42"
 DIMS = (256, 256, 256)
 OMEGA, LAMBDA, XFORCE = 1.2, 0.1875, 1e-6
-B_EVEN, B_ODD = 2 * 19 * 8, 2 * 19 * 8 + 18 * 4
+B_EVEN = 2 * 19 * 8
 PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "list_odd_porous256_traffic.json")
 
@@ -108,14 +113,19 @@ def main():
     t = k.timings()
     loop_s = t.loopMs / 1e3
     value = nf * steps / loop_s / 1e6
-    odd_gbs = B_ODD * nf / (t.oddMsAvg / 1e3) / 1e9
+    b_odd = info.bytesPerFlupOdd
+    odd_gbs = b_odd * nf / (t.oddMsAvg / 1e3) / 1e9
     even_gbs = B_EVEN * nf / (t.evenMsAvg / 1e3) / 1e9
     print(f"bench: {value:.1f} MFLUP/s loop {t.loopMs:.2f} ms wall {wall * 1e3:.2f} ms even {t.evenMsAvg:.4f} ms "
           f"({even_gbs:.0f} GB/s) odd {t.oddMsAvg:.4f} ms ({odd_gbs:.0f} GB/s) runs={info.nRuns}",
           file=sys.stderr, flush=True)
     mask = geo.mask.reshape(-1).copy()
     periodic = geo.periodic
+    runs_wave = info.nRunsWave
     k.close()
+    yard = lbm.stream_bench(1 << 27, nf, reps=10)
+    print(f"bench: yardsticks copy {yard['copy_gbs']:.0f} GB/s update-19 {yard['update19_gbs']:.0f} GB/s",
+          file=sys.stderr, flush=True)
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -144,13 +154,16 @@ def main():
         "config": {"workload": "list-aa-pv-soa, 256x256x256 porous medium (spheres r=6, porosity<=0.80, seed 42), "
                                "x periodic, omega 1.2, Lambda 3/16, Luo force 1e-6, FP64",
                    "n_fluid": nf, "n_cells_padded": info.nCells, "device": dev.get("name")},
-        "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step)", "achieved": round(odd_gbs, 1),
-                     "peak": PEAK_GBS, "unit": "GB/s", "frac": round(odd_gbs / PEAK_GBS, 4),
-                     "traffic": traffic, "bytes_per_flup": B_ODD,
+        "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step, run-compressed adjacency)",
+                     "achieved": round(odd_gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_flup": round(b_odd, 3), "odd_ms_avg": t.oddMsAvg, "runs_per_wave_cut": runs_wave,
                      "even_kernel": {"achieved": round(even_gbs, 1), "frac": round(even_gbs / PEAK_GBS, 4),
                                      "bytes_per_flup": B_EVEN, "ms_avg": t.evenMsAvg},
-                     "odd_ms_avg": t.oddMsAvg,
-                     "step_average_frac": round((B_EVEN + B_ODD) / 2 * nf * steps / loop_s / 1e9 / PEAK_GBS, 4)},
+                     "step_average_gbs": round((B_EVEN + b_odd) / 2 * nf * steps / loop_s / 1e9, 1),
+                     "yardsticks_gbs": {"copy": round(yard["copy_gbs"], 1), "update19": round(yard["update19_gbs"], 1)},
+                     "frac_of_copy": round(odd_gbs / yard["copy_gbs"], 4),
+                     "list_aa_340B_equivalent_frac": round(340 * nf * steps / loop_s / 1e9 / PEAK_GBS, 4)},
         "cpu_baseline": base,
     }
     os.dup2(json_fd, 1)

@@ -119,6 +119,19 @@ int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, 
 int LbmHipDumpNodes(KernelData * kd, const LatticeDesc * ld, PdfT * out);
 int LbmHipLoadNodes(KernelData * kd, const LatticeDesc * ld, const PdfT * in);
 
+/* Streaming-bandwidth yardsticks on this device, the way the reference
+ * rates its kernels against copy and "update-19" bandwidth
+ * (doc/main.rst:508-511): a copy of nDoubles doubles into a second array,
+ * and an in-place read-modify-write of 19 arrays of nNodes doubles each.
+ * Results in GB/s (1e9 B/s), averaged over reps launches after warm-up. */
+typedef struct {
+	double copyGBs;
+	double update19GBs;
+	double copyBytes;        /* bytes read + written by one copy launch */
+	double update19Bytes;    /* bytes read + written by one update-19 launch */
+} LbmHipStreamResult;
+int LbmHipStreamBench(uint64_t nDoubles, uint32_t nNodes, int reps, LbmHipStreamResult * out);
+
 /* Name and compute-unit count of the device the library runs on. */
 int LbmHipDeviceInfo(char * name, int nameLen, int * computeUnits, double * memClockGHz);
 

@@ -310,7 +310,10 @@ int LbmHipDeviceInfo(char * name, int nameLen, int * cus, double * memClockGHz)
 	if (hipGetDevice(&dev) != hipSuccess) return -1;
 	hipDeviceProp_t p;
 	if (hipGetDeviceProperties(&p, dev) != hipSuccess) return -1;
-	if (name && nameLen > 0) snprintf(name, (size_t)nameLen, "%s (%s)", p.name, p.gcnArchName);
+	if (name && nameLen > 0) {
+		if (p.name[0]) snprintf(name, (size_t)nameLen, "%s (%s)", p.name, p.gcnArchName);
+		else snprintf(name, (size_t)nameLen, "%s, %d CUs", p.gcnArchName, p.multiProcessorCount);
+	}
 	if (cus) *cus = p.multiProcessorCount;
 	if (memClockGHz) *memClockGHz = p.memoryClockRate / 1.0e6;
 	return 0;
@@ -404,3 +407,67 @@ int LbmHipLoadNodes(KernelData * kd, const LatticeDesc * ld, const PdfT * in)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- stream yardsticks
+namespace lbm {
+
+__global__ __launch_bounds__(256) void k_stream_copy(double * __restrict__ dst, const double * __restrict__ src, uint64_t n)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) dst[i] = src[i];
+}
+
+// 19 planes read and written in place, the memory pattern of an even step
+__global__ __launch_bounds__(256) void k_stream_update19(double * p, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	double v[19];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) v[d] = p[(size_t)d * n + i];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) p[(size_t)d * n + i] = v[d] * 0.5 + 1.0;
+}
+
+}  // namespace lbm
+
+extern "C" int LbmHipStreamBench(uint64_t nDoubles, uint32_t nNodes, int reps, LbmHipStreamResult * out)
+{
+	using namespace lbm;
+	if (!out || nDoubles == 0 || nNodes == 0 || reps < 1) { fail("LbmHipStreamBench: invalid arguments."); return -1; }
+	double * a = nullptr;
+	double * b = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = -1;
+	do {
+		const size_t bytesA = sizeof(double) * (size_t)(nDoubles > 19ull * nNodes ? nDoubles : 19ull * nNodes);
+		if (!hip_ok(hipMalloc((void **)&a, bytesA), "hipMalloc")) break;
+		if (!hip_ok(hipMalloc((void **)&b, sizeof(double) * (size_t)nDoubles), "hipMalloc")) break;
+		if (!hip_ok(hipMemset(a, 0, bytesA), "hipMemset")) break;
+		if (!hip_ok(hipEventCreate(&e0), "hipEventCreate") || !hip_ok(hipEventCreate(&e1), "hipEventCreate")) break;
+		const unsigned gc = (unsigned)((nDoubles + 255) / 256), gu = (nNodes + 255) / 256;
+		float ms = 0.0f;
+		(void)hipGetLastError();
+		for (int i = 0; i < 2; ++i) k_stream_copy<<<gc, 256>>>(b, a, nDoubles);
+		if (!hip_ok(hipEventRecord(e0), "hipEventRecord")) break;
+		for (int i = 0; i < reps; ++i) k_stream_copy<<<gc, 256>>>(b, a, nDoubles);
+		if (!hip_ok(hipEventRecord(e1), "hipEventRecord") || !hip_ok(hipEventSynchronize(e1), "copy kernel")) break;
+		if (!hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) break;
+		out->copyBytes = 16.0 * (double)nDoubles;
+		out->copyGBs = out->copyBytes * reps / (ms * 1e-3) / 1e9;
+		for (int i = 0; i < 2; ++i) k_stream_update19<<<gu, 256>>>(a, nNodes);
+		if (!hip_ok(hipEventRecord(e0), "hipEventRecord")) break;
+		for (int i = 0; i < reps; ++i) k_stream_update19<<<gu, 256>>>(a, nNodes);
+		if (!hip_ok(hipEventRecord(e1), "hipEventRecord") || !hip_ok(hipEventSynchronize(e1), "update-19 kernel")) break;
+		if (!hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) break;
+		if (!hip_ok(hipGetLastError(), "stream kernels")) break;
+		out->update19Bytes = 2.0 * 19 * 8 * (double)nNodes;
+		out->update19GBs = out->update19Bytes * reps / (ms * 1e-3) / 1e9;
+		rc = 0;
+	} while (0);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipFree(a);
+	(void)hipFree(b);
+	return rc;
+}

@@ -85,6 +85,11 @@ class LbmHipListInfo(C.Structure):
                 ("nRunsWave", C.c_int), ("adjRle", C.c_int)]
 
 
+class LbmHipStreamResult(C.Structure):
+    _fields_ = [("copyGBs", C.c_double), ("update19GBs", C.c_double), ("copyBytes", C.c_double),
+                ("update19Bytes", C.c_double)]
+
+
 class GeoPorous(C.Structure):
     _fields_ = [("Radius", C.c_double), ("Porosity", C.c_double), ("Seed", C.c_uint64),
                 ("nSpheres", C.c_int)]
@@ -99,7 +104,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "g_lbmHipKernelCount", "LbmHipSetErrorMode", "LbmHipLastError", "LbmHipClearError",
                "LbmHipGetTimings", "LbmHipSetLaunchEvents", "LbmHipSetShadowDownload",
                "LbmHipSyncShadow", "LbmHipListGetInfo", "LbmHipListCopyBuild", "LbmHipDumpNodes",
-               "LbmHipLoadNodes", "LbmHipDeviceInfo"]
+               "LbmHipLoadNodes", "LbmHipDeviceInfo", "LbmHipStreamBench"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -157,6 +162,7 @@ def hip_lib() -> C.CDLL:
         L.LbmHipDumpNodes.argtypes = [KD, C.POINTER(LatticeDesc), C.c_void_p]
         L.LbmHipLoadNodes.argtypes = [KD, C.POINTER(LatticeDesc), C.c_void_p]
         L.LbmHipDeviceInfo.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.LbmHipStreamBench.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(LbmHipStreamResult)]
         _libs["hip"] = L
     return _libs["hip"]
 
@@ -167,6 +173,15 @@ def kernel_table() -> dict:
     n = C.c_int.in_dll(L, "g_lbmHipKernelCount").value
     tab = (KernelFunctions * n).in_dll(L, "g_lbmHipKernels")
     return {tab[i].Name.decode(): tab[i] for i in range(n)}
+
+
+def stream_bench(n_doubles: int, n_nodes: int, reps: int = 10) -> dict:
+    """Copy and update-19 bandwidth yardsticks of the device (GB/s)."""
+    r = LbmHipStreamResult()
+    if hip_lib().LbmHipStreamBench(n_doubles, n_nodes, reps, C.byref(r)) != 0:
+        raise RuntimeError(hip_lib().LbmHipLastError().decode())
+    return {"copy_gbs": r.copyGBs, "update19_gbs": r.update19GBs, "copy_bytes": r.copyBytes,
+            "update19_bytes": r.update19Bytes}
 
 
 def device_info():
