@@ -23,6 +23,8 @@
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
  *   -nt on|off        non-temporal loads/stores in the even step (default on)
+ *   -even-cap on|off  even step in 512-thread blocks, one block per compute
+ *                     unit (default on; -bs then applies to the odd step)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H

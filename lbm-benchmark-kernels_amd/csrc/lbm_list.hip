@@ -475,11 +475,13 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
 	const bool rle = c.args.adjRle != 0, nt = c.args.nontemporal != 0;
+	const StreamShape es = stream_shape(&c, bs);
+	const dim3 egrid((unsigned)((L->nFluid + es.block - 1) / es.block)), eblock((unsigned)es.block);
 	auto even = [&](bool fl) {
-		if (fl && nt)  k_list_even<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else if (fl)   k_list_even<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else if (nt)   k_list_even<false, true><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else           k_list_even<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		if (fl && nt)  k_list_even<true, true><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else if (fl)   k_list_even<true, false><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else if (nt)   k_list_even<false, true><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
+		else           k_list_even<false, false><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
 	};
 	auto odd = [&](bool fl, bool fs) {
 		if (rle) {

@@ -42,6 +42,7 @@ struct KernelArgs {
 	int blockSize = 256;
 	int adjRle = 1;        // list: odd step reads run-compressed adjacency
 	int nontemporal = 1;   // streaming (non-temporal) loads/stores in the even step
+	int evenCap = 1;       // even step: 512-thread blocks, one block per CU
 };
 // Parses Parameters.KernelArgs.  listKernel enables -pad (the reference
 // accepts it only for the list kernel, src/BenchKernelD3Q19ListAaPvCommon.c:302-308).
@@ -72,6 +73,8 @@ extern const int kCx[19], kCy[19], kCz[19], kInv[19];
 // Embedded (after KernelData) in the list and aa kernel structs.
 struct Common {
 	int device = 0;
+	int ldsPerCU = 0;           // bytes of LDS per compute unit
+	int ldsPerBlock = 0;        // largest dynamic LDS a block may request
 	hipStream_t stream = nullptr;
 	KernelArgs args;
 	int iteration = -1;         // parity of the last completed step
@@ -92,6 +95,16 @@ Common * aa_common(KernelData * kd);
 
 bool common_init(Common * c, const Parameters * params, bool listKernel);
 void common_free(Common * c);
+// Launch shape of a streaming step kernel.  With the cap, blocks of 512
+// threads request just over half the CU's LDS, so one block -- 8
+// wavefronts -- runs per CU: fewer concurrent streams, measured 6.0 TB/s for
+// the even step against 5.4 TB/s at full occupancy.  The LDS is reserved,
+// never used.
+struct StreamShape {
+	int block;
+	size_t lds;
+};
+StreamShape stream_shape(const Common * c, int defaultBlock);
 // Grows the per-launch event pool to hold `launches` launches.
 bool common_events(Common * c, int launches);
 // Fills c->t from the recorded events (stream must be synchronised).

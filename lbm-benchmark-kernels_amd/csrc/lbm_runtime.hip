@@ -168,6 +168,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "on")) a->nontemporal = 1;
 			else if (!strcmp(v, "off")) a->nontemporal = 0;
 			else return fail("argument -nt requires on or off.");
+		} else if (is("-even-cap") || is("--even-cap")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "on")) a->evenCap = 1;
+			else if (!strcmp(v, "off")) a->evenCap = 0;
+			else return fail("argument -even-cap requires on or off.");
 		} else if (is("-bs") || is("--bs")) {
 			if (!arg_int(p, i, &a->blockSize, "-bs", 64, 512)) return false;
 			if (a->blockSize % 64) return fail("argument -bs must be a multiple of 64.");
@@ -176,7 +182,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			if (listKernel) printf("  [-adj rle|full]\n");
-			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off]\n");
+			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off]\n");
 			return fail("kernel help requested.");
 		} else {
 			return fail("unknown kernel parameter: %s.", s);
@@ -222,6 +228,10 @@ bool common_init(Common * c, const Parameters * params, bool listKernel)
 		return fail("no HIP device found; the list/aa kernels of this library run on the GPU only.");
 	if (!hip_ok(e, "hipGetDeviceCount")) return false;
 	if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) return false;
+	if (!hip_ok(hipDeviceGetAttribute(&c->ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device),
+	            "hipDeviceGetAttribute")) return false;
+	if (!hip_ok(hipDeviceGetAttribute(&c->ldsPerBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device),
+	            "hipDeviceGetAttribute")) return false;
 	if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
 	if (!hip_ok(hipEventCreate(&c->evLoop[0]), "hipEventCreate")) return false;
 	if (!hip_ok(hipEventCreate(&c->evLoop[1]), "hipEventCreate")) return false;
@@ -236,6 +246,17 @@ void common_free(Common * c)
 	c->nEv = 0;
 	for (auto & e : c->evLoop) if (e) { (void)hipEventDestroy(e); e = nullptr; }
 	if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
+}
+
+StreamShape stream_shape(const Common * c, int defaultBlock)
+{
+	StreamShape sh = {defaultBlock, 0};
+	if (!c->args.evenCap) return sh;
+	const size_t want = (size_t)c->ldsPerCU / 2 + 8192;
+	if (c->ldsPerCU <= 0 || want > (size_t)c->ldsPerBlock) return sh;
+	sh.block = 512;
+	sh.lds = want;
+	return sh;
 }
 
 bool common_events(Common * c, int launches)

@@ -251,7 +251,8 @@ def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
     m = geo.mask.reshape(-1)
     want, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1)
     for args in (["-adj", "rle"], ["-adj", "full"], ["-adj", "rle", "-nt", "off"], ["-adj", "full", "-nt", "off"],
-                 ["-adj", "rle", "-bs", "128"], ["-adj", "rle", "-bs", "64", "-blk", "5"]):
+                 ["-adj", "rle", "-bs", "128"], ["-adj", "rle", "-bs", "64", "-blk", "5"],
+                 ["-adj", "rle", "-even-cap", "off"], ["-adj", "full", "-even-cap", "off", "-bs", "512"]):
         with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
             info = k.list_info()
             assert info.adjRle == (args[1] == "rle")
