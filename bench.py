@@ -13,6 +13,10 @@ even/odd pairs), measured with HIP events recorded on the kernel's stream
 around the step loop and around every launch.  value = nFluid * steps /
 time / 1e6 (src/Main.c:594).
 
+--prec fp32s / fp16s run the shifted-storage variants (deviations f - w in
+FP32 or scaled binary16, collision in FP32) on the same workload: 4 or 2 B
+per value instead of 8; the default and headline is FP64.
+
 roofline: the odd-step kernel (the longer of the two).  Algorithmic bytes
 per fluid node: 19 loads + 19 stores of 8 B = 304 B plus the adjacency it
 reads -- run-compressed by default: 72 B per run of consecutive adjacency
@@ -42,7 +46,6 @@ sys.path.insert(0, os.path.join(ROOT, "lbm-benchmark-kernels_amd"))
 GEOMETRY = "porous-6-0.8-42"
 DIMS = (256, 256, 256)
 OMEGA, LAMBDA, XFORCE = 1.2, 0.1875, 1e-6
-B_EVEN = 2 * 19 * 8
 PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "list_odd_porous256_traffic.json")
 
@@ -81,10 +84,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--prec", choices=["fp64", "fp32s", "fp16s"], default="fp64",
+                    help="PDF storage: fp64 (the headline), or deviations f - w in FP32 / scaled FP16")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU; runs on several GPUs are not part of this build "
-              "(the multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU (its share of the lattice at the real shape); "
+              "runs on several GPUs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     # the C libraries print their reports on stdout; keep stdout for the JSON line
@@ -99,8 +104,9 @@ def main():
     geo = lbm.Geometry.from_string(GEOMETRY, DIMS)
     nf = geo.n_fluid
     print(f"bench: {GEOMETRY} {DIMS} nFluid={nf} steps={steps} warmup={warm}", file=sys.stderr, flush=True)
-    k = lbm.Kernel("list-aa-pv-soa", geo, ["-lambda", repr(LAMBDA)])
+    k = lbm.Kernel("list-aa-pv-soa", geo, ["-lambda", repr(LAMBDA), "-prec", args.prec])
     info = k.list_info()
+    b_even = info.bytesPerFlupEven
     k.set_initial_weights()
     k.set_shadow_download(False)
     cd = lbm.case_data(omega=OMEGA, xforce=XFORCE, iterations=warm)
@@ -115,7 +121,7 @@ def main():
     value = nf * steps / loop_s / 1e6
     b_odd = info.bytesPerFlupOdd
     odd_gbs = b_odd * nf / (t.oddMsAvg / 1e3) / 1e9
-    even_gbs = B_EVEN * nf / (t.evenMsAvg / 1e3) / 1e9
+    even_gbs = b_even * nf / (t.evenMsAvg / 1e3) / 1e9
     print(f"bench: {value:.1f} MFLUP/s loop {t.loopMs:.2f} ms wall {wall * 1e3:.2f} ms even {t.evenMsAvg:.4f} ms "
           f"({even_gbs:.0f} GB/s) odd {t.oddMsAvg:.4f} ms ({odd_gbs:.0f} GB/s) runs={info.nRuns}",
           file=sys.stderr, flush=True)
@@ -128,7 +134,7 @@ def main():
           file=sys.stderr, flush=True)
 
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if args.prec == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
@@ -138,8 +144,9 @@ def main():
         base = cpu_baseline(mask, DIMS, periodic, nf, args.cpu_steps + (args.cpu_steps & 1))
 
     dev = lbm.device_info() or {}
+    label = {"fp64": "FP64", "fp32s": "FP32 shifted storage", "fp16s": "FP16 shifted storage, FP32 compute"}[args.prec]
     line = {
-        "metric": "MFLUP/s (D3Q19 TRT list-AA FP64, 256^3 porous)",
+        "metric": f"MFLUP/s (D3Q19 TRT list-AA {label}, 256^3 porous)",
         "value": round(value, 2),
         "unit": "MFLUP/s",
         "n_gpus": 1,
@@ -149,18 +156,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32s": "f32", "fp16s": "f16 storage / f32 compute"}[args.prec],
         "data": "synthetic",
         "config": {"workload": "list-aa-pv-soa, 256x256x256 porous medium (spheres r=6, porosity<=0.80, seed 42), "
-                               "x periodic, omega 1.2, Lambda 3/16, Luo force 1e-6, FP64",
+                               f"x periodic, omega 1.2, Lambda 3/16, Luo force 1e-6, {label}",
                    "n_fluid": nf, "n_cells_padded": info.nCells, "device": dev.get("name")},
         "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step, run-compressed adjacency)",
                      "achieved": round(odd_gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_flup": round(b_odd, 3), "odd_ms_avg": t.oddMsAvg, "runs_per_wave_cut": runs_wave,
                      "even_kernel": {"achieved": round(even_gbs, 1), "frac": round(even_gbs / PEAK_GBS, 4),
-                                     "bytes_per_flup": B_EVEN, "ms_avg": t.evenMsAvg},
-                     "step_average_gbs": round((B_EVEN + b_odd) / 2 * nf * steps / loop_s / 1e9, 1),
+                                     "bytes_per_flup": b_even, "ms_avg": t.evenMsAvg},
+                     "step_average_gbs": round((b_even + b_odd) / 2 * nf * steps / loop_s / 1e9, 1),
                      "yardsticks_gbs": {"copy": round(yard["copy_gbs"], 1), "update19": round(yard["update19_gbs"], 1)},
                      "frac_of_copy": round(odd_gbs / yard["copy_gbs"], 4),
                      "list_aa_340B_equivalent_frac": round(340 * nf * steps / loop_s / 1e9 / PEAK_GBS, 4)},

@@ -25,6 +25,19 @@
  *   -nt on|off        non-temporal loads/stores in the even step (default on)
  *   -even-cap on|off  even step in 512-thread blocks, one block per compute
  *                     unit (default on; -bs then applies to the odd step)
+ *   -prec fp64|fp32s|fp16s
+ *                     list only: PDF storage.  fp64 (default) is the
+ *                     reference's PdfT = double.  fp32s/fp16s store the
+ *                     deviation f - w_d (FP16: times 2^k, see -fp16-scale)
+ *                     and collide in FP32; no counterpart in the reference,
+ *                     whose single precision stores f in float
+ *                     (src/Kernel.h:75-76).  GetNode/SetNode still exchange
+ *                     double node views; KernelData.Pdfs[0] then points to
+ *                     the storage type.  The body-force increment is added
+ *                     in FP32 at the next step's load (after the last step of
+ *                     a Kernel() call, at the store).
+ *   -fp16-scale <2^k> list only: power-of-two factor of the FP16 deviations
+ *                     (default 4096)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H
@@ -106,8 +119,32 @@ typedef struct {
 	double bytesPerFlupOdd;    /* algorithmic bytes per node of the odd step as configured */
 	int nRunsWave;             /* runs when cut at every 64-node wavefront (run-compressed odd step) */
 	int adjRle;                /* 1: odd step reads the run-compressed adjacency (-adj rle) */
+	int prec;                  /* PDF storage (-prec): 0 fp64, 1 fp32s, 2 fp16s */
 } LbmHipListInfo;
 int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out);
+
+/* FP64 sums over the fluid nodes of the current lattice, computed on the
+ * device: mass = sum_n sum_d f_d and momentumX = sum_n sum_d c_x,d f_d
+ * (node views; the counterparts of the per-node loops of KernelDensity and
+ * KernelStatisticsAdv, src/Kernel.c:209-254, 633-755).  Valid between
+ * Kernel() calls; pending SetNode changes are uploaded first. */
+int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX);
+
+/* Runs the case in Kernel() calls of checkEvery steps (rounded up to even)
+ * until the relative change of the x momentum between two checks is <= tol
+ * or maxSteps steps have run.  The host shadow is refreshed once at the end
+ * (when shadow download is on); kd->Duration is the summed step time.
+ * No counterpart in the reference, which always runs MaxIterations steps
+ * (src/Main.c:574). */
+typedef struct {
+	int steps;          /* steps run */
+	int converged;      /* 1 when the tolerance was met */
+	double relChange;   /* last relative change of momentumX (-1 before two checks) */
+	double mass;        /* LbmHipListMoments after the last chunk */
+	double momentumX;
+	double seconds;     /* summed step-loop time */
+} LbmHipSteady;
+int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out);
 
 /* Copies the device list build to caller buffers, in the reference's
  * layouts (any pointer may be NULL):

@@ -122,7 +122,7 @@ __device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int 
 // of the same cell (src/BenchKernelD3Q19Aa.c:146-290).
 template <bool FORCE_LOAD>
 __global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
-                                                 TrtConsts k, ForceTerms ft)
+                                                 TrtC<double> k, ForceT<double> ft)
 {
 	int x, y, z;
 	if (!aa_cell(g, x, y, z)) return;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * 
 // here too.
 template <bool FORCE_LOAD, bool FORCE_STORE>
 __global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
-                                                TrtConsts k, ForceTerms ft)
+                                                TrtC<double> k, ForceT<double> ft)
 {
 	int x, y, z;
 	if (!aa_cell(g, x, y, z)) return;
@@ -297,8 +297,8 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 		c.t.uploadMs = 0.0;
 	}
 
-	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
-	const ForceTerms ft = force_terms(cd->XForce);
+	const TrtC<double> k = trt_cast<double>(trt_consts(cd->Omega, c.args.magic));
+	const ForceT<double> ft = force_cast<double>(force_terms(cd->XForce));
 	const bool force = c.args.forceOn != 0;
 	const bool perLaunch = c.launchEvents != 0;
 	if (perLaunch && !common_events(&c, steps)) return;

@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_fp16.h>
+
 #include "lbm_runtime.h"
 
 namespace lbm {
@@ -45,29 +47,62 @@ __host__ __device__ constexpr int cz(int d)
 	       (d == dB || d == dBS || d == dBN || d == dBW || d == dBE) ? -1 : 0;
 }
 
+// TRT constants in the compute precision R (values rounded from the
+// double constants of TrtConsts).
+template <class R>
+struct TrtC {
+	R oe, oo, w0, w1, w2, w1x3, w1nh, w2x3, w2nh;
+};
+
+template <class R>
+inline TrtC<R> trt_cast(const TrtConsts & k)
+{
+	return TrtC<R>{(R)k.oe, (R)k.oo, (R)k.w0, (R)k.w1, (R)k.w2, (R)k.w1x3, (R)k.w1nh, (R)k.w2x3, (R)k.w2nh};
+}
+
+template <class R>
+struct ForceT {
+	R f[19];
+};
+
+template <class R>
+inline ForceT<R> force_cast(const ForceTerms & t)
+{
+	ForceT<R> r;
+	for (int d = 0; d < 19; ++d) r.f[d] = (R)t.f[d];
+	return r;
+}
+
 // In place: f[] holds the 19 pre-collision values on entry and the
 // post-collision value of each direction (not yet swapped) on exit.
-__device__ __forceinline__ void trt_collide(double (&f)[19], const TrtConsts & k)
+//
+// The same routine serves shifted storage: applied to the deviations
+// s_d = f_d - w_d (rho0 = 1) it yields the post-collision deviations
+// exactly, because the incompressible equilibrium is linear in the density
+// (the velocity is sum c_d f_d without division by rho) and sum w_d = 1:
+// the rho0 terms cancel in the rest, even and odd parts alike.
+template <class R>
+__device__ __forceinline__ void trt_collide(R (&f)[19], const TrtC<R> & k)
 {
-	const double ux = f[dE] + f[dNE] + f[dSE] + f[dTE] + f[dBE] - f[dW] - f[dNW] - f[dSW] - f[dTW] - f[dBW];
-	const double uy = f[dN] + f[dNE] + f[dNW] + f[dTN] + f[dBN] - f[dS] - f[dSE] - f[dSW] - f[dTS] - f[dBS];
-	const double uz = f[dT] + f[dTE] + f[dTW] + f[dTN] + f[dTS] - f[dB] - f[dBE] - f[dBW] - f[dBN] - f[dBS];
-	const double dens = f[dC] + f[dN] + f[dE] + f[dS] + f[dW] + f[dNE] + f[dSE] + f[dSW] + f[dNW]
-	                  + f[dT] + f[dTN] + f[dTE] + f[dTS] + f[dTW] + f[dB] + f[dBN] + f[dBE] + f[dBS] + f[dBW];
-	const double dir = dens - (ux * ux + uy * uy + uz * uz) * 3.0 / 2.0;
+	const R ux = f[dE] + f[dNE] + f[dSE] + f[dTE] + f[dBE] - f[dW] - f[dNW] - f[dSW] - f[dTW] - f[dBW];
+	const R uy = f[dN] + f[dNE] + f[dNW] + f[dTN] + f[dBN] - f[dS] - f[dSE] - f[dSW] - f[dTS] - f[dBS];
+	const R uz = f[dT] + f[dTE] + f[dTW] + f[dTN] + f[dTS] - f[dB] - f[dBE] - f[dBW] - f[dBN] - f[dBS];
+	const R dens = f[dC] + f[dN] + f[dE] + f[dS] + f[dW] + f[dNE] + f[dSE] + f[dSW] + f[dNW]
+	             + f[dT] + f[dTN] + f[dTE] + f[dTS] + f[dTW] + f[dB] + f[dBN] + f[dBE] + f[dBS] + f[dBW];
+	const R dir = dens - (ux * ux + uy * uy + uz * uz) * R(3.0) / R(2.0);
 
 	f[dC] = f[dC] - k.oe * (f[dC] - k.w0 * dir);
 
-	const double w1i = k.w1 * dir;
-	const double w2i = k.w2 * dir;
+	const R w1i = k.w1 * dir;
+	const R w2i = k.w2 * dir;
 
 	// One opposite pair (a, b) with projected velocity ui: the even part
 	// relaxes with omegaEven, the odd part with omegaOdd.
-	auto pair = [&](int a, int b, double ui, double wi, double wnh, double wx3) {
-		const double ev = k.oe * (0.5 * (f[a] + f[b]) - ui * ui * wnh - wi);
-		const double od = k.oo * (0.5 * (f[a] - f[b]) - ui * wx3);
-		const double pa = f[a] - ev - od;
-		const double pb = f[b] - ev + od;
+	auto pair = [&](int a, int b, R ui, R wi, R wnh, R wx3) {
+		const R ev = k.oe * (R(0.5) * (f[a] + f[b]) - ui * ui * wnh - wi);
+		const R od = k.oo * (R(0.5) * (f[a] - f[b]) - ui * wx3);
+		const R pa = f[a] - ev - od;
+		const R pb = f[b] - ev + od;
 		f[a] = pa;
 		f[b] = pb;
 	};
@@ -81,5 +116,50 @@ __device__ __forceinline__ void trt_collide(double (&f)[19], const TrtConsts & k
 	pair(dTS, dBN, -uy + uz, w2i, k.w2nh, k.w2x3);
 	pair(dTN, dBS, uy + uz,  w2i, k.w2nh, k.w2x3);
 }
+
+}  // namespace lbm
+
+namespace lbm {
+
+// Storage codecs of the PDF array.  load/store convert between the stored
+// value and the compute precision R; NT selects non-temporal accesses.
+struct CodecF64 {               // the reference's FP64 values
+	using S = double;
+	using R = double;
+	__device__ __forceinline__ R load(const S * p, bool nt) const { return nt ? __builtin_nontemporal_load(p) : *p; }
+	__device__ __forceinline__ void store(S * p, R v, bool nt) const
+	{
+		if (nt) __builtin_nontemporal_store(v, p);
+		else *p = v;
+	}
+};
+
+struct CodecF32 {               // deviation f - w in FP32
+	using S = float;
+	using R = float;
+	__device__ __forceinline__ R load(const S * p, bool nt) const { return nt ? __builtin_nontemporal_load(p) : *p; }
+	__device__ __forceinline__ void store(S * p, R v, bool nt) const
+	{
+		if (nt) __builtin_nontemporal_store(v, p);
+		else *p = v;
+	}
+};
+
+struct CodecF16 {               // deviation f - w times a power of two, in FP16
+	using S = uint16_t;
+	using R = float;
+	float scale, inv;            // 2^k and 2^-k: the scaling itself is exact
+	__device__ __forceinline__ R load(const S * p, bool nt) const
+	{
+		const uint16_t h = nt ? __builtin_nontemporal_load(p) : *p;
+		return __half2float(__ushort_as_half(h)) * inv;
+	}
+	__device__ __forceinline__ void store(S * p, R v, bool nt) const
+	{
+		const uint16_t h = __half_as_ushort(__float2half_rn(v * scale));
+		if (nt) __builtin_nontemporal_store(h, p);
+		else *p = h;
+	}
+};
 
 }  // namespace lbm

@@ -51,8 +51,10 @@ struct ListData {
 	uint32_t * dGrid = nullptr;
 	uint32_t * dCoords = nullptr;
 	uint32_t * dAdj = nullptr;
-	double * dPdf = nullptr;
-	double * hPdf = nullptr;     // host shadow, == kd.Pdfs[0]
+	void * dPdf = nullptr;       // 19 * nCells values of the storage type
+	void * hPdf = nullptr;       // host shadow, same layout and type, == kd.Pdfs[0]
+	double * dMoments = nullptr; // reduction scratch: 2 per block + 2
+	int momentBlocks = 0;
 	uint32_t * hGrid = nullptr;  // cell -> node id, for GetNode/SetNode
 };
 
@@ -307,26 +309,24 @@ struct AdjRuns {
 // increment of the preceding odd step to each loaded value.
 // NT streams the lattice with non-temporal loads and stores: every value is
 // touched once per step, so caching it only evicts useful lines.
-template <bool FORCE_LOAD, bool NT>
-__global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid, uint32_t nCells,
-                                                   TrtConsts k, ForceTerms ft)
+// Codec: storage type and compute precision (FP64 values, or deviations
+// f - w in FP32 / scaled FP16 computed in FP32).
+template <class Codec, bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_list_even(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
+                                                   TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
+	using R = typename Codec::R;
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
-	double f[19];
+	R f[19];
 #pragma unroll
 	for (int d = 0; d < 19; ++d) {
-		double * p = &pdf[(size_t)d * nCells + n];
-		f[d] = NT ? __builtin_nontemporal_load(p) : *p;
+		f[d] = cod.load(&pdf[(size_t)d * nCells + n], NT);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	trt_collide(f, k);
 #pragma unroll
-	for (int d = 0; d < 19; ++d) {
-		double * p = &pdf[(size_t)inv_dir(d) * nCells + n];
-		if (NT) __builtin_nontemporal_store(f[d], p);
-		else *p = f[d];
-	}
+	for (int d = 0; d < 19; ++d) cod.store(&pdf[(size_t)inv_dir(d) * nCells + n], f[d], NT);
 }
 
 // Odd step: direction d arrives from slot adj[n][inv d] (the neighbour's
@@ -336,34 +336,94 @@ __global__ __launch_bounds__(512) void k_list_even(double * pdf, uint32_t nFluid
 // race free.  FORCE_STORE adds the force increment of this step to each
 // stored value: a value stored through a bounce-back entry is read back by
 // the same node as direction inv(d), any other by the neighbour as d.
-template <bool FORCE_LOAD, bool FORCE_STORE, class Adj>
-__global__ __launch_bounds__(512) void k_list_odd(double * pdf, Adj adjacency, uint32_t nFluid, uint32_t nCells,
-                                                  TrtConsts k, ForceTerms ft)
+template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, class Adj>
+__global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec cod, Adj adjacency, uint32_t nFluid,
+                                                  uint32_t nCells, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
+	using R = typename Codec::R;
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	uint32_t a[18];
 	adjacency.fetch(n, a);
-	double f[19];
+	R f[19];
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
-		f[d] = pdf[a[inv_dir(d)]];
+		f[d] = cod.load(&pdf[a[inv_dir(d)]], false);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	const size_t cSlot = (size_t)dC * nCells + n;
-	f[dC] = pdf[cSlot];
+	f[dC] = cod.load(&pdf[cSlot], false);
 	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
 	trt_collide(f, k);
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
-		double v = f[d];
+		R v = f[d];
 		if (FORCE_STORE) {
 			const bool bounce = a[d] == (uint32_t)inv_dir(d) * nCells + n;
 			v = v + (bounce ? ft.f[inv_dir(d)] : ft.f[d]);
 		}
-		pdf[a[d]] = v;
+		cod.store(&pdf[a[d]], v, false);
 	}
-	pdf[cSlot] = FORCE_STORE ? f[dC] + ft.f[dC] : f[dC];
+	cod.store(&pdf[cSlot], FORCE_STORE ? f[dC] + ft.f[dC] : f[dC], false);
+}
+
+// Node sums in the local layout (after a completed Kernel() call), in FP64:
+// per block the sum over its nodes of sum_d v_d and of sum_d c_x,d v_d, v
+// the stored value (f, or the deviation f - w for shifted storage); a second
+// launch adds the block sums in a fixed order, so results do not depend on
+// timing.
+template <class Codec>
+__global__ __launch_bounds__(256) void k_list_moments(const typename Codec::S * pdf, Codec cod, uint32_t nFluid,
+                                                      uint32_t nCells, double * __restrict__ partial)
+{
+	const uint32_t n = blockIdx.x * 256 + threadIdx.x;
+	double m = 0.0, jx = 0.0;
+	if (n < nFluid) {
+		for (int d = 0; d < 19; ++d) {
+			const double v = (double)cod.load(&pdf[(size_t)d * nCells + n], false);
+			m += v;
+			jx += cx(d) * v;
+		}
+	}
+	__shared__ double sm[256], sj[256];
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		partial[2 * blockIdx.x] = sm[0];
+		partial[2 * blockIdx.x + 1] = sj[0];
+	}
+}
+
+__global__ __launch_bounds__(256) void k_sum_pairs(const double * __restrict__ partial, uint32_t nBlocks, double * __restrict__ out)
+{
+	__shared__ double sm[256], sj[256];
+	double m = 0.0, jx = 0.0;
+	for (uint32_t i = threadIdx.x; i < nBlocks; i += 256) {
+		m += partial[2 * i];
+		jx += partial[2 * i + 1];
+	}
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		out[0] = sm[0];
+		out[1] = sj[0];
+	}
 }
 
 // --------------------------------------------------------- node access
@@ -375,6 +435,35 @@ static bool list_local_layout(const ListData * L)
 {
 	if (L->c.iteration % 2 != 0) return true;
 	return fail("list-aa-pv-soa: node access after an even step is not supported by this build.");
+}
+
+// Shadow element <-> node-view value.  Shifted storage holds f - w_d
+// (w_d = w_inv(d), so a slot keeps the same shift whichever node reads it).
+static const double kW[19] = {
+	1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+	1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+	1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 3.0 };
+
+static size_t elem_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32S ? 4 : 2; }
+
+static double shadow_get(const ListData * L, size_t i, int d)
+{
+	switch (L->c.args.prec) {
+		case PREC_FP32S: return kW[d] + (double)((const float *)L->hPdf)[i];
+		case PREC_FP16S: return kW[d] + (double)(f16_to_f32(((const uint16_t *)L->hPdf)[i]) / (float)(1 << L->c.args.f16Shift));
+		default: return ((const double *)L->hPdf)[i];
+	}
+}
+
+static void shadow_set(ListData * L, size_t i, int d, double v)
+{
+	switch (L->c.args.prec) {
+		case PREC_FP32S: ((float *)L->hPdf)[i] = (float)(v - kW[d]); break;
+		case PREC_FP16S:
+			((uint16_t *)L->hPdf)[i] = f32_to_f16((float)(v - kW[d]) * (float)(1 << L->c.args.f16Shift));
+			break;
+		default: ((double *)L->hPdf)[i] = v;
+	}
 }
 
 // Node id of a fluid cell, or false (with the error recorded) for a cell
@@ -395,7 +484,7 @@ static void list_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 	ListData * L = as_list(kd);
 	uint32_t n;
 	if (!list_node(L, x, y, z, &n)) return;
-	for (int d = 0; d < 19; ++d) pdfs[d] = L->hPdf[(size_t)d * L->nCells + n];
+	for (int d = 0; d < 19; ++d) pdfs[d] = shadow_get(L, (size_t)d * L->nCells + n, d);
 }
 
 static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
@@ -403,7 +492,7 @@ static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 	ListData * L = as_list(kd);
 	uint32_t n;
 	if (!list_node(L, x, y, z, &n)) return;
-	for (int d = 0; d < 19; ++d) L->hPdf[(size_t)d * L->nCells + n] = pdfs[d];
+	for (int d = 0; d < 19; ++d) shadow_set(L, (size_t)d * L->nCells + n, d, pdfs[d]);
 	L->c.shadowDirty = 1;
 }
 
@@ -412,7 +501,7 @@ static void list_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pd
 	ListData * L = as_list(kd);
 	uint32_t n;
 	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19) return;
-	*pdf = L->hPdf[(size_t)dir * L->nCells + n];
+	*pdf = shadow_get(L, (size_t)dir * L->nCells + n, dir);
 }
 
 static void list_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
@@ -420,7 +509,7 @@ static void list_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 	ListData * L = as_list(kd);
 	uint32_t n;
 	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19) return;
-	L->hPdf[(size_t)dir * L->nCells + n] = pdf;
+	shadow_set(L, (size_t)dir * L->nCells + n, dir, pdf);
 	L->c.shadowDirty = 1;
 }
 
@@ -429,7 +518,7 @@ static bool list_upload(ListData * L)
 {
 	if (!L->c.shadowDirty) { L->c.t.uploadMs = 0.0; return true; }
 	const double t0 = (double)clock() / CLOCKS_PER_SEC;
-	if (!hip_ok(hipMemcpyAsync(L->dPdf, L->hPdf, sizeof(double) * 19 * (size_t)L->nCells,
+	if (!hip_ok(hipMemcpyAsync(L->dPdf, L->hPdf, elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells,
 	                           hipMemcpyHostToDevice, L->c.stream), "upload of the host lattice")) return false;
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize")) return false;
 	L->c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
@@ -440,7 +529,7 @@ static bool list_upload(ListData * L)
 static bool list_download(ListData * L)
 {
 	if (L->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
-	if (!hip_ok(hipMemcpyAsync(L->hPdf, L->dPdf, sizeof(double) * 19 * (size_t)L->nCells,
+	if (!hip_ok(hipMemcpyAsync(L->hPdf, L->dPdf, elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells,
 	                           hipMemcpyDeviceToHost, L->c.stream), "download of the device lattice")) return false;
 	return hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize");
 }
@@ -450,53 +539,94 @@ static bool list_download(ListData * L)
 // 4 B plus 12 B of mask and base per 64-node wavefront.
 static double list_odd_bytes(const ListData * L)
 {
-	if (!L->c.args.adjRle || L->nFluid == 0) return 2.0 * 19 * 8 + 18 * 4.0;
-	return 2.0 * 19 * 8 + (72.0 * L->nRunsWave + 12.0 * L->nWaves) / L->nFluid;
+	const double pdf = 2.0 * 19 * (double)elem_bytes(L->c.args.prec);
+	if (!L->c.args.adjRle || L->nFluid == 0) return pdf + 18 * 4.0;
+	return pdf + (72.0 * L->nRunsWave + 12.0 * L->nWaves) / L->nFluid;
 }
 
-static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+static double list_even_bytes(const ListData * L) { return 2.0 * 19 * (double)elem_bytes(L->c.args.prec); }
+
+struct LaunchCfg {
+	dim3 egrid, eblock;
+	size_t elds;
+	dim3 grid, block;
+	hipStream_t stream;
+	uint32_t nF, nC;
+	bool rle, nt;
+};
+
+template <class Codec>
+static void launch_even(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, const TrtConsts & k0,
+                        const ForceTerms & f0, bool fl)
 {
-	(void)ld;
+	using R = typename Codec::R;
+	const TrtC<R> k = trt_cast<R>(k0);
+	const ForceT<R> ft = force_cast<R>(f0);
+	if (fl && c.nt)  k_list_even<Codec, true, true><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	else if (fl)     k_list_even<Codec, true, false><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	else if (c.nt)   k_list_even<Codec, false, true><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	else             k_list_even<Codec, false, false><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+}
+
+template <class Codec, class Adj>
+static void launch_odd_adj(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, Adj adj, const TrtC<typename Codec::R> & k,
+                           const ForceT<typename Codec::R> & ft, bool fl, bool fs)
+{
+	if (fs)      k_list_odd<Codec, true, true><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
+	else if (fl) k_list_odd<Codec, true, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
+	else         k_list_odd<Codec, false, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
+}
+
+template <class Codec>
+static void launch_odd(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, const AdjFull & full, const AdjRuns & runs,
+                       const TrtConsts & k0, const ForceTerms & f0, bool fl, bool fs)
+{
+	using R = typename Codec::R;
+	const TrtC<R> k = trt_cast<R>(k0);
+	const ForceT<R> ft = force_cast<R>(f0);
+	if (c.rle) launch_odd_adj(c, pdf, cod, runs, k, ft, fl, fs);
+	else       launch_odd_adj(c, pdf, cod, full, k, ft, fl, fs);
+}
+
+static bool list_steps(KernelData * kd, const CaseData * cd)
+{
 	ListData * L = as_list(kd);
 	Common & c = L->c;
 	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
-	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { fail("omega must be in (0, 2), got %f.", cd->Omega); return; }
-	if (!list_upload(L)) return;
+	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { return fail("omega must be in (0, 2), got %f.", cd->Omega); }
+	if (!list_upload(L)) return false;
 
 	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
 	const ForceTerms ft = force_terms(cd->XForce);
 	const bool force = c.args.forceOn != 0;
 	const bool perLaunch = c.launchEvents != 0;
-	if (perLaunch && !common_events(&c, steps)) return;
+	if (perLaunch && !common_events(&c, steps)) return false;
 
 	const int bs = c.args.blockSize;
 	const dim3 grid((unsigned)((L->nFluid + bs - 1) / bs)), block((unsigned)bs);
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
-	const bool rle = c.args.adjRle != 0, nt = c.args.nontemporal != 0;
 	const StreamShape es = stream_shape(&c, bs);
-	const dim3 egrid((unsigned)((L->nFluid + es.block - 1) / es.block)), eblock((unsigned)es.block);
+	const LaunchCfg cfg = {dim3((unsigned)((L->nFluid + es.block - 1) / es.block)), dim3((unsigned)es.block), es.lds,
+	                       grid, block, c.stream, nF, nC, c.args.adjRle != 0, c.args.nontemporal != 0};
+	const float f16s = (float)(1 << c.args.f16Shift);
 	auto even = [&](bool fl) {
-		if (fl && nt)  k_list_even<true, true><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else if (fl)   k_list_even<true, false><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else if (nt)   k_list_even<false, true><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
-		else           k_list_even<false, false><<<egrid, eblock, es.lds, c.stream>>>(L->dPdf, nF, nC, k, ft);
-	};
-	auto odd = [&](bool fl, bool fs) {
-		if (rle) {
-			if (fs)      k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
-			else if (fl) k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
-			else         k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, runs, nF, nC, k, ft);
-		} else {
-			if (fs)      k_list_odd<true, true><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
-			else if (fl) k_list_odd<true, false><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
-			else         k_list_odd<false, false><<<grid, block, 0, c.stream>>>(L->dPdf, full, nF, nC, k, ft);
+		switch (c.args.prec) {
+			case PREC_FP32S: launch_even(cfg, (float *)L->dPdf, CodecF32{}, k, ft, fl); break;
+			case PREC_FP16S: launch_even(cfg, (uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, fl); break;
+			default:         launch_even(cfg, (double *)L->dPdf, CodecF64{}, k, ft, fl);
 		}
 	};
-
+	auto odd = [&](bool fl, bool fs) {
+		switch (c.args.prec) {
+			case PREC_FP32S: launch_odd(cfg, (float *)L->dPdf, CodecF32{}, full, runs, k, ft, fl, fs); break;
+			case PREC_FP16S: launch_odd(cfg, (uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, full, runs, k, ft, fl, fs); break;
+			default:         launch_odd(cfg, (double *)L->dPdf, CodecF64{}, full, runs, k, ft, fl, fs);
+		}
+	};
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
-	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
+	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
 	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
@@ -508,9 +638,9 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 		odd(force, force && last);
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
 	}
-	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
-	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
-	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
+	if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
+	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return false;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return false;
 
 	common_collect(&c, L->nFluid > 0 ? steps : 0, perLaunch);
 	c.t.steps = steps;
@@ -519,9 +649,46 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 
 	if (c.shadowDownload) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!list_download(L)) return;
+		if (!list_download(L)) return false;
 		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
 	}
+	return true;
+}
+
+static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+{
+	(void)ld;
+	(void)list_steps(kd, cd);
+}
+
+// Mass sum_n sum_d f and x momentum sum_n sum_d c_x,d f of the device
+// lattice, in FP64 (shifted storage: the stored deviations plus the weights,
+// which add 1 per node to the mass and nothing to the momentum).
+static bool list_moments(ListData * L, double * mass, double * jx)
+{
+	Common & c = L->c;
+	*mass = 0.0;
+	*jx = 0.0;
+	if (!list_upload(L)) return false;
+	if (L->nFluid == 0) return true;
+	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells, nb = (uint32_t)L->momentBlocks;
+	double * part = L->dMoments, * sum = L->dMoments + 2 * (size_t)nb;
+	const float f16s = (float)(1 << c.args.f16Shift);
+	switch (c.args.prec) {
+		case PREC_FP32S: k_list_moments<<<nb, 256, 0, c.stream>>>((const float *)L->dPdf, CodecF32{}, nF, nC, part); break;
+		case PREC_FP16S:
+			k_list_moments<<<nb, 256, 0, c.stream>>>((const uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, nF, nC, part);
+			break;
+		default: k_list_moments<<<nb, 256, 0, c.stream>>>((const double *)L->dPdf, CodecF64{}, nF, nC, part);
+	}
+	k_sum_pairs<<<1, 256, 0, c.stream>>>(part, nb, sum);
+	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
+	double h[2];
+	if (!hip_ok(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, c.stream), "moment download")) return false;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "moment kernels")) return false;
+	*mass = h[0] + (c.args.prec == PREC_FP64 ? 0.0 : (double)L->nFluid);
+	*jx = h[1];
+	return true;
 }
 
 // ------------------------------------------------------------ Init/Deinit
@@ -535,6 +702,7 @@ static void list_release(ListData * L)
 	(void)hipFree(L->dRunBase);
 	(void)hipFree(L->dRunAdj);
 	(void)hipFree(L->dPdf);
+	(void)hipFree(L->dMoments);
 	host_free(L->hPdf);
 	free(L->hGrid);
 	common_free(&L->c);
@@ -689,22 +857,27 @@ void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Pa
 
 	if (!list_build(L, ld)) { list_release(L); return; }
 
-	const size_t pdfBytes = sizeof(double) * 19 * (size_t)L->nCells;
+	// zero bits are 0.0 in every storage type: for shifted storage they mean
+	// f = w, for FP64 f = 0 as the reference initialises
+	// (src/BenchKernelD3Q19ListAaPvCommon.c:583-585)
+	const size_t pdfBytes = elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells;
 	if (!dev_alloc((void **)&L->dPdf, pdfBytes, "device lattice")) { list_release(L); return; }
 	if (!hip_ok(hipMemsetAsync(L->dPdf, 0, pdfBytes, L->c.stream), "hipMemset")) { list_release(L); return; }
-	L->hPdf = (double *)host_alloc(pdfBytes);
+	L->momentBlocks = (L->nFluid + 255) / 256;
+	if (!dev_alloc((void **)&L->dMoments, sizeof(double) * (2 * (size_t)L->momentBlocks + 2), "moment scratch")) { list_release(L); return; }
+	L->hPdf = host_alloc(pdfBytes);
 	if (!L->hPdf) { fail("allocating host lattice failed: %zu bytes.", pdfBytes); list_release(L); return; }
 	memset(L->hPdf, 0, pdfBytes);
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "init")) { list_release(L); return; }
 
-	const double even = 2.0 * 19 * 8, odd = list_odd_bytes(L);
+	const double even = list_even_bytes(L), odd = list_odd_bytes(L);
 	printf("# total fluid nodes: %d   consecutive blocks: %d   runs cut at wavefronts: %d\n",
 	       L->nFluid, L->nRuns, L->nRunsWave);
 	printf("# loop balance:          %.2f B/FLUP  even: %.2f B/FLUP  odd %.2f B/FLUP\n", (even + odd) / 2, even, odd);
 
 	KernelData * kd = &L->kd;
-	kd->Pdfs[0] = L->hPdf;
-	kd->PdfsActive = L->hPdf;
+	kd->Pdfs[0] = (PdfT *)L->hPdf;     // storage type given by -prec (double by default)
+	kd->PdfsActive = (PdfT *)L->hPdf;
 	for (int i = 0; i < 3; ++i) { kd->Dims[i] = ld->Dims[i]; kd->GlobalDims[i] = ld->Dims[i]; kd->Offsets[i] = 0; }
 	kd->LoopBalance = (even + odd) / 2;
 	kd->GetNode = list_get_node;
@@ -734,11 +907,61 @@ int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out)
 	out->adjStride = L->adjStride;
 	out->nRuns = L->nRuns;
 	for (int i = 0; i < 3; ++i) { out->dims[i] = L->dims[i]; out->periodic[i] = L->periodic[i]; }
-	out->bytesPerFlupEven = 2.0 * 19 * 8;
+	out->bytesPerFlupEven = list_even_bytes(L);
+	out->prec = L->c.args.prec;
 	out->bytesPerFlupOdd = list_odd_bytes(L);
 	out->nRunsWave = L->nRunsWave;
 	out->adjRle = L->c.args.adjRle;
 	return 0;
+}
+
+int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX)
+{
+	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListMoments: not a list-aa-pv-soa kernel."); return -1; }
+	double m = 0.0, j = 0.0;
+	if (!list_moments(as_list(kd), &m, &j)) return -1;
+	if (mass) *mass = m;
+	if (momentumX) *momentumX = j;
+	return 0;
+}
+
+int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out)
+{
+	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListRunSteady: not a list-aa-pv-soa kernel."); return -1; }
+	if (!cd || checkEvery <= 0 || maxSteps <= 0 || !(tol >= 0.0)) {
+		fail("LbmHipListRunSteady: needs a case, checkEvery > 0, maxSteps > 0 and tol >= 0.");
+		return -1;
+	}
+	ListData * L = as_list(kd);
+	Common & c = L->c;
+	const int every = checkEvery + (checkEvery & 1);
+	const int saveDownload = c.shadowDownload;
+	c.shadowDownload = 0;
+	LbmHipSteady r = {};
+	r.relChange = -1.0;
+	double prev = 0.0, seconds = 0.0;
+	bool ok = true, havePrev = false;
+	while (r.steps < maxSteps) {
+		CaseData chunk = *cd;
+		chunk.MaxIterations = every < maxSteps - r.steps ? every : maxSteps - r.steps;
+		if (!list_steps(kd, &chunk)) { ok = false; break; }
+		r.steps += chunk.MaxIterations + (chunk.MaxIterations & 1);
+		seconds += kd->Duration;
+		if (!list_moments(L, &r.mass, &r.momentumX)) { ok = false; break; }
+		if (havePrev) {
+			const double d = fabs(r.momentumX - prev), a = fabs(r.momentumX);
+			r.relChange = a > 0.0 ? d / a : (d > 0.0 ? INFINITY : 0.0);
+			if (r.relChange <= tol) { r.converged = 1; break; }
+		}
+		prev = r.momentumX;
+		havePrev = true;
+	}
+	c.shadowDownload = saveDownload;
+	kd->Duration = seconds;
+	r.seconds = seconds;
+	if (ok && saveDownload && !list_download(L)) ok = false;
+	if (out) *out = r;
+	return ok ? 0 : -1;
 }
 
 int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, uint32_t * coords)

@@ -43,7 +43,17 @@ struct KernelArgs {
 	int adjRle = 1;        // list: odd step reads run-compressed adjacency
 	int nontemporal = 1;   // streaming (non-temporal) loads/stores in the even step
 	int evenCap = 1;       // even step: 512-thread blocks, one block per CU
+	int prec = 0;          // list: PDF storage, see enum Prec
+	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
 };
+
+// PDF storage of the list kernel.
+enum Prec { PREC_FP64 = 0, PREC_FP32S = 1, PREC_FP16S = 2 };
+
+// IEEE binary16 conversions on the host, round to nearest even (the
+// rounding of the device's v_cvt_f16_f32).
+uint16_t f32_to_f16(float f);
+float f16_to_f32(uint16_t h);
 // Parses Parameters.KernelArgs.  listKernel enables -pad (the reference
 // accepts it only for the list kernel, src/BenchKernelD3Q19ListAaPvCommon.c:302-308).
 bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel);

@@ -168,6 +168,20 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "on")) a->nontemporal = 1;
 			else if (!strcmp(v, "off")) a->nontemporal = 0;
 			else return fail("argument -nt requires on or off.");
+		} else if (listKernel && (is("-prec") || is("--prec"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -prec requires fp64, fp32s or fp16s.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "fp64")) a->prec = PREC_FP64;
+			else if (!strcmp(v, "fp32s")) a->prec = PREC_FP32S;
+			else if (!strcmp(v, "fp16s")) a->prec = PREC_FP16S;
+			else return fail("argument -prec requires fp64, fp32s or fp16s.");
+		} else if (listKernel && (is("-fp16-scale") || is("--fp16-scale"))) {
+			int v;
+			if (!arg_int(p, i, &v, "-fp16-scale", 1, 1 << 24)) return false;
+			if (v & (v - 1)) return fail("argument -fp16-scale must be a power of two.");
+			int k = 0;
+			while ((1 << k) < v) ++k;
+			a->f16Shift = k;
 		} else if (is("-even-cap") || is("--even-cap")) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on or off.");
 			const char * v = p->KernelArgs[++i];
@@ -181,7 +195,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			if (listKernel) printf("  [-adj rle|full]\n");
+			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off]\n");
 			return fail("kernel help requested.");
 		} else {
@@ -189,6 +203,47 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 		}
 	}
 	return true;
+}
+
+// ---------------------------------------------------------------- binary16
+uint16_t f32_to_f16(float f)
+{
+	uint32_t x;
+	memcpy(&x, &f, 4);
+	const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+	if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+	if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);          // rounds past 65504
+	if (ax >= 0x38800000u) {                                             // normal result
+		uint32_t h = ((((ax >> 23) - 127u + 15u)) << 10) | ((ax & 0x7fffffu) >> 13);
+		const uint32_t rem = ax & 0x1fffu;
+		if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;       // carries into the exponent
+		return (uint16_t)(sign | h);
+	}
+	if (ax <= 0x33000000u) return (uint16_t)sign;                       // <= 2^-25: ties to zero
+	const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | 0x800000u;
+	const uint32_t shift = 126u - e;                                     // subnormal: m * 2^(e-150) / 2^-24
+	uint32_t h = m >> shift;
+	const uint32_t rem = m & ((1u << shift) - 1u), halfway = 1u << (shift - 1u);
+	if (rem > halfway || (rem == halfway && (h & 1u))) ++h;
+	return (uint16_t)(sign | h);
+}
+
+float f16_to_f32(uint16_t h)
+{
+	const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+	uint32_t x;
+	if (e == 0x1fu) x = sign | 0x7f800000u | (m << 13);
+	else if (e != 0) x = sign | ((e + 112u) << 23) | (m << 13);
+	else if (m == 0) x = sign;
+	else {
+		int k = 0;
+		uint32_t mm = m;
+		while (!(mm & 0x400u)) { mm <<= 1; ++k; }
+		x = sign | ((uint32_t)(113 - k) << 23) | ((mm & 0x3ffu) << 13);
+	}
+	float f;
+	memcpy(&f, &x, 4);
+	return f;
 }
 
 // ---------------------------------------------------------------- physics

@@ -82,7 +82,12 @@ class LbmHipListInfo(C.Structure):
     _fields_ = [("nFluid", C.c_int), ("nCells", C.c_int), ("adjStride", C.c_int), ("nRuns", C.c_int),
                 ("dims", C.c_int * 3), ("periodic", C.c_int * 3),
                 ("bytesPerFlupEven", C.c_double), ("bytesPerFlupOdd", C.c_double),
-                ("nRunsWave", C.c_int), ("adjRle", C.c_int)]
+                ("nRunsWave", C.c_int), ("adjRle", C.c_int), ("prec", C.c_int)]
+
+
+class LbmHipSteady(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("relChange", C.c_double),
+                ("mass", C.c_double), ("momentumX", C.c_double), ("seconds", C.c_double)]
 
 
 class LbmHipStreamResult(C.Structure):
@@ -104,7 +109,8 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "g_lbmHipKernelCount", "LbmHipSetErrorMode", "LbmHipLastError", "LbmHipClearError",
                "LbmHipGetTimings", "LbmHipSetLaunchEvents", "LbmHipSetShadowDownload",
                "LbmHipSyncShadow", "LbmHipListGetInfo", "LbmHipListCopyBuild", "LbmHipDumpNodes",
-               "LbmHipLoadNodes", "LbmHipDeviceInfo", "LbmHipStreamBench"]
+               "LbmHipLoadNodes", "LbmHipDeviceInfo", "LbmHipStreamBench", "LbmHipListMoments",
+               "LbmHipListRunSteady"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -163,6 +169,9 @@ def hip_lib() -> C.CDLL:
         L.LbmHipLoadNodes.argtypes = [KD, C.POINTER(LatticeDesc), C.c_void_p]
         L.LbmHipDeviceInfo.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.LbmHipStreamBench.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(LbmHipStreamResult)]
+        L.LbmHipListMoments.argtypes = [KD, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.LbmHipListRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(LbmHipSteady)]
         _libs["hip"] = L
     return _libs["hip"]
 
@@ -360,6 +369,21 @@ class Kernel:
         if hip_lib().LbmHipListCopyBuild(self.kd, adj.ctypes.data, grid.ctypes.data, coords.ctypes.data) != 0:
             raise RuntimeError(hip_lib().LbmHipLastError().decode())
         return adj, grid, coords
+
+    def moments(self):
+        """(mass, x momentum) of the device lattice, FP64 sums on the GPU."""
+        m, j = C.c_double(), C.c_double()
+        if hip_lib().LbmHipListMoments(self.kd, C.byref(m), C.byref(j)) != 0:
+            raise RuntimeError(hip_lib().LbmHipLastError().decode())
+        return m.value, j.value
+
+    def run_steady(self, cd: CaseData, check_every: int, tol: float, max_steps: int) -> LbmHipSteady:
+        r = LbmHipSteady()
+        L = hip_lib()
+        L.LbmHipClearError()
+        if L.LbmHipListRunSteady(self.kd, C.byref(cd), check_every, tol, max_steps, C.byref(r)) != 0:
+            raise RuntimeError(L.LbmHipLastError().decode())
+        return r
 
     def close(self):
         if self.kd:

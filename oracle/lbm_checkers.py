@@ -26,6 +26,7 @@ NLINK = 18
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 def _i3(v):
@@ -99,6 +100,12 @@ class Oracle:
         L.orc_force_term.restype = C.c_double
         L.orc_verify_norm.argtypes = [_f64p, _i32p, C.c_int, C.c_int, C.c_double, C.c_double]
         L.orc_verify_norm.restype = C.c_double
+        L.orc_f16_round.argtypes = [C.c_float]
+        L.orc_f16_round.restype = C.c_float
+        L.orc_list_run_shifted.argtypes = [C.POINTER(_ListT), _f32p, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_shifted_from_nodes.argtypes = [C.POINTER(_ListT), _f32p, C.c_int, C.c_int, _f64p]
+        L.orc_shifted_to_nodes.argtypes = [C.POINTER(_ListT), _f32p, C.c_int, C.c_int, _f64p]
         self.CX = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CX"))
         self.CY = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CY"))
         self.CZ = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CZ"))
@@ -191,6 +198,41 @@ class Oracle:
                               n_threads, vsize)
         out = np.zeros((b.n_fluid, NDIR))
         self.lib.orc_list_dump(C.byref(st), out)
+        return out, b
+
+    PREC = {"fp32s": 1, "fp16s": 2}
+
+    def f16_round(self, v: float) -> float:
+        return float(self.lib.orc_f16_round(v))
+
+    def list_run_shifted(self, mask, dims, periodic, runs, omega, xforce, prec, *, f16_shift=12,
+                         magic=1.0 / 12.0, force_on=True, blk=(0, 0, 0), nodes=None,
+                         build: ListBuild | None = None):
+        """Shifted-storage list-AA (prec "fp32s" or "fp16s").  ``runs`` is the
+        step count of each Kernel() call, run back to back; ``nodes`` the
+        initial node views in mask order (default f = w).  Returns the node
+        views after the last run (nFluid x 19, mask order) and the build."""
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        b = build or self.list_build(mask, dims, periodic, blk)
+        code = self.PREC[prec]
+        store = np.zeros(NDIR * b.n_cells, dtype=np.float32)
+        adj = np.ascontiguousarray(b.adj.reshape(-1))
+        st = _ListT()
+        st.dims = _i3(dims)
+        st.nFluid, st.nCells = b.n_fluid, b.n_cells
+        st.mask = mask.ctypes.data
+        st.grid = b.grid.ctypes.data
+        st.adj = adj.ctypes.data
+        st.iteration = -1
+        if nodes is None:
+            nodes = np.tile(self.W, (b.n_fluid, 1))
+        self.lib.orc_shifted_from_nodes(C.byref(st), store, code, f16_shift,
+                                        np.ascontiguousarray(nodes, dtype=np.float64))
+        for steps in ([runs] if isinstance(runs, int) else runs):
+            self.lib.orc_list_run_shifted(C.byref(st), store, code, f16_shift, int(steps), omega,
+                                          magic, xforce, int(force_on))
+        out = np.zeros((b.n_fluid, NDIR))
+        self.lib.orc_shifted_to_nodes(C.byref(st), store, code, f16_shift, out)
         return out, b
 
     def aa_run(self, mask, dims, periodic, iterations, omega, xforce, *, magic=1.0 / 12.0,

@@ -672,3 +672,165 @@ double orc_verify_norm(const double * ux, const int * fluid, int nY, int nZ,
 	}
 	return sqrt(dev);
 }
+
+/* ------------------------------------------------------------------------
+ * Shifted storage, FP32 and scaled FP16.  No counterpart in the reference:
+ * its single-precision build keeps f itself in float (src/Kernel.h:75-76).
+ * Every slot holds the deviation s = f - w_d (w_d = w_inv(d), so a value
+ * keeps its shift whichever direction reads it), the collision runs in
+ * float on the deviations with the scalar tree of orc_trt_collide, and the
+ * body-force increment is added in float when the next step loads a value,
+ * or, after the last odd step of a run, when it is stored.  This schedule
+ * is the definition of the variants: in FP32 it equals adding the increment
+ * to the stored values after every step (same operands, same rounding); in
+ * FP16 it avoids rounding the increment into binary16 every step.
+ *
+ * FP16 slots are modelled by their values: the float array holds
+ * s * 2^k rounded to the nearest binary16 value, ties to even
+ * (IEEE 754-2019 roundTiesToEven), and is read back as value * 2^-k.
+ * --------------------------------------------------------------------- */
+float orc_f16_round(float v)
+{
+	const double a = fabs((double)v);
+	if (a == 0.0 || a != a || a > 1e300) return v;
+	int e;
+	(void)frexp(a, &e);                 /* a = m 2^e, m in [0.5, 1): exponent e - 1 */
+	int ee = e - 1;
+	if (ee < -14) ee = -14;             /* subnormal binary16: fixed quantum 2^-24 */
+	const int q = ee - 10;              /* 10 fraction bits */
+	const double r = ldexp(nearbyint(ldexp(a, -q)), q);   /* exact scaling, current rounding = to nearest even */
+	if (r > 65504.0) return v < 0 ? -INFINITY : INFINITY;
+	return (float)(v < 0 ? -r : r);
+}
+
+typedef struct {
+	float oe, oo, w0, w1, w2, w1x3, w1nh, w2x3, w2nh;
+} trt_f;
+
+static trt_f trt_f_consts(double omega, double magic)
+{
+	/* each constant computed in double and rounded once */
+	const double w1 = 1.0 / 18.0, w2 = 1.0 / 36.0;
+	trt_f k = { (float)omega, (float)orc_omega_odd(omega, magic), (float)(1.0 / 3.0), (float)w1, (float)w2,
+	            (float)(w1 * 3.0), (float)(w1 * 9.0 / 2.0), (float)(w2 * 3.0), (float)(w2 * 9.0 / 2.0) };
+	return k;
+}
+
+/* orc_trt_collide's scalar tree in float (src/BenchKernelD3Q19ListAaPv.c:387-463) */
+static void trt_collide_f(const float p[ORC_NDIR], float q[ORC_NDIR], const trt_f * k)
+{
+	const float ux = p[E_] + p[NE_] + p[SE_] + p[TE_] + p[BE_] - p[W_] - p[NW_] - p[SW_] - p[TW_] - p[BW_];
+	const float uy = p[N_] + p[NE_] + p[NW_] + p[TN_] + p[BN_] - p[S_] - p[SE_] - p[SW_] - p[TS_] - p[BS_];
+	const float uz = p[T_] + p[TE_] + p[TW_] + p[TN_] + p[TS_] - p[B_] - p[BE_] - p[BW_] - p[BN_] - p[BS_];
+	const float dens = p[C_] + p[N_] + p[E_] + p[S_] + p[W_] + p[NE_] + p[SE_] + p[SW_] + p[NW_]
+	                 + p[T_] + p[TN_] + p[TE_] + p[TS_] + p[TW_] + p[B_] + p[BN_] + p[BE_] + p[BS_] + p[BW_];
+	const float dir = dens - (ux * ux + uy * uy + uz * uz) * 3.0f / 2.0f;
+
+	q[C_] = p[C_] - k->oe * (p[C_] - k->w0 * dir);
+	const float w1i = k->w1 * dir, w2i = k->w2 * dir;
+	float ev, od, ui;
+#define PAIR(a, b, uiexpr, wi, wnh, wx3)                               \
+	ui = (uiexpr);                                                     \
+	ev = k->oe * (0.5f * (p[a] + p[b]) - ui * ui * (wnh) - (wi));      \
+	od = k->oo * (0.5f * (p[a] - p[b]) - ui * (wx3));                  \
+	q[a] = p[a] - ev - od;                                             \
+	q[b] = p[b] - ev + od;
+
+	PAIR(N_,  S_,  uy,        w1i, k->w1nh, k->w1x3)
+	PAIR(E_,  W_,  ux,        w1i, k->w1nh, k->w1x3)
+	PAIR(T_,  B_,  uz,        w1i, k->w1nh, k->w1x3)
+	PAIR(NW_, SE_, -ux + uy,  w2i, k->w2nh, k->w2x3)
+	PAIR(NE_, SW_, ux + uy,   w2i, k->w2nh, k->w2x3)
+	PAIR(TW_, BE_, -ux + uz,  w2i, k->w2nh, k->w2x3)
+	PAIR(TE_, BW_, ux + uz,   w2i, k->w2nh, k->w2x3)
+	PAIR(TS_, BN_, -uy + uz,  w2i, k->w2nh, k->w2x3)
+	PAIR(TN_, BS_, uy + uz,   w2i, k->w2nh, k->w2x3)
+#undef PAIR
+}
+
+typedef struct {
+	int half;
+	float scale, inv;
+} codec_f;
+
+static codec_f make_codec(int prec, int f16Shift)
+{
+	codec_f c = { prec == ORC_PREC_FP16S, 1.0f, 1.0f };
+	if (c.half) { c.scale = ldexpf(1.0f, f16Shift); c.inv = ldexpf(1.0f, -f16Shift); }
+	return c;
+}
+static inline float cload(const codec_f * c, float x) { return c->half ? x * c->inv : x; }
+static inline float cstore(const codec_f * c, float v) { return c->half ? orc_f16_round(v * c->scale) : v; }
+
+void orc_list_run_shifted(const orc_list_t * L, float * st, int prec, int f16Shift, int iterations,
+                          double omega, double magic, double F, int forceOn)
+{
+	const trt_f k = trt_f_consts(omega, magic);
+	const codec_f cd = make_codec(prec, f16Shift);
+	const size_t nc = (size_t)L->nCells;
+	const int nF = L->nFluid;
+	float ft[ORC_NDIR];
+	for (int d = 0; d < ORC_NDIR; ++d) ft[d] = (float)orc_force_term(d, F);
+
+	for (int it = 0; it < iterations; it += 2) {
+		const int loadForce = forceOn && it > 0, last = it + 2 >= iterations;
+		#pragma omp parallel for schedule(static)
+		for (int n = 0; n < nF; ++n) {
+			float p[ORC_NDIR], q[ORC_NDIR];
+			for (int d = 0; d < ORC_NDIR; ++d) {
+				p[d] = cload(&cd, st[(size_t)d * nc + n]);
+				if (loadForce) p[d] = p[d] + ft[d];
+			}
+			trt_collide_f(p, q, &k);
+			for (int d = 0; d < ORC_NDIR; ++d) st[(size_t)ORC_INV[d] * nc + n] = cstore(&cd, q[d]);
+		}
+		#pragma omp parallel for schedule(static)
+		for (int n = 0; n < nF; ++n) {
+			float p[ORC_NDIR], q[ORC_NDIR];
+			const uint32_t * a = L->adj + (size_t)n * ORC_NLINK;
+			for (int d = 0; d < ORC_NLINK; ++d) p[d] = cload(&cd, st[a[ORC_INV[d]]]);
+			p[C_] = cload(&cd, st[(size_t)C_ * nc + n]);
+			if (forceOn) for (int d = 0; d < ORC_NDIR; ++d) p[d] = p[d] + ft[d];
+			trt_collide_f(p, q, &k);
+			for (int d = 0; d < ORC_NLINK; ++d) {
+				float v = q[d];
+				if (forceOn && last) {
+					/* a bounce-back store is read back by this node as inv(d) */
+					const int bb = a[d] == (uint32_t)((size_t)ORC_INV[d] * nc + n);
+					v = v + ft[bb ? ORC_INV[d] : d];
+				}
+				st[a[d]] = cstore(&cd, v);
+			}
+			st[(size_t)C_ * nc + n] = cstore(&cd, (forceOn && last) ? q[C_] + ft[C_] : q[C_]);
+		}
+	}
+}
+
+/* Node views of complete step pairs (local layout), in mask order. */
+void orc_shifted_from_nodes(const orc_list_t * L, float * st, int prec, int f16Shift, const double * nodes)
+{
+	const codec_f cd = make_codec(prec, f16Shift);
+	const int X = L->dims[0], Y = L->dims[1], Z = L->dims[2];
+	size_t i = 0;
+	for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+		if (L->mask[lidx(L->dims, x, y, z)] == 0) continue;
+		const uint32_t n = L->grid[lidx(L->dims, x, y, z)];
+		for (int d = 0; d < ORC_NDIR; ++d)
+			st[(size_t)d * L->nCells + n] = cstore(&cd, (float)(nodes[i * ORC_NDIR + d] - ORC_W[d]));
+		++i;
+	}
+}
+
+void orc_shifted_to_nodes(const orc_list_t * L, const float * st, int prec, int f16Shift, double * nodes)
+{
+	const codec_f cd = make_codec(prec, f16Shift);
+	const int X = L->dims[0], Y = L->dims[1], Z = L->dims[2];
+	size_t i = 0;
+	for (int z = 0; z < Z; ++z) for (int y = 0; y < Y; ++y) for (int x = 0; x < X; ++x) {
+		if (L->mask[lidx(L->dims, x, y, z)] == 0) continue;
+		const uint32_t n = L->grid[lidx(L->dims, x, y, z)];
+		for (int d = 0; d < ORC_NDIR; ++d)
+			nodes[i * ORC_NDIR + d] = ORC_W[d] + (double)cload(&cd, st[(size_t)d * L->nCells + n]);
+		++i;
+	}
+}

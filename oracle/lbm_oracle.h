@@ -163,6 +163,19 @@ double orc_verify_norm(const double * uxSlice, const int * fluidSlice, int nY, i
 /* Canonical node view: every fluid cell in mask order (x fastest), 19 values
  * each, taken through the parity-aware accessors. */
 void orc_list_dump(const orc_list_t * L, double * out);
+
+/* ---- shifted storage (no reference counterpart) ----------------------
+ * Slots hold f - w_d in float; for FP16S the float holds (f - w_d) * 2^k
+ * rounded to the nearest binary16 value.  Collision in float, force
+ * increments added at the next load (at the store after the last odd
+ * step).  Node views are valid between runs (complete step pairs). */
+#define ORC_PREC_FP32S 1
+#define ORC_PREC_FP16S 2
+float orc_f16_round(float v);
+void orc_list_run_shifted(const orc_list_t * L, float * store, int prec, int f16Shift, int iterations,
+                          double omega, double magic, double xForce, int forceOn);
+void orc_shifted_from_nodes(const orc_list_t * L, float * store, int prec, int f16Shift, const double * nodes);
+void orc_shifted_to_nodes(const orc_list_t * L, const float * store, int prec, int f16Shift, double * nodes);
 void orc_aa_dump(const orc_aa_t * A, double * out);
 
 #ifdef __cplusplus

@@ -49,3 +49,21 @@ def golden(name: str):
 
 def tiny_cases():
     return sorted(f[5:-4] for f in os.listdir(GOLDEN) if f.startswith("tiny_") and f.endswith(".npz"))
+
+
+def blk_args(g):
+    """Kernel arguments stored in a tiny fixture (node-numbering blocks)."""
+    return [str(a) for a in g["kernel_args"] if str(a)]
+
+
+def blk_tuple(args):
+    """The (bx, by, bz) of -blk / -blk-x/y/z words, 0 = unblocked."""
+    blk = [0, 0, 0]
+    it = iter(args)
+    for a in it:
+        v = int(next(it))
+        if a == "-blk":
+            blk = [v, v, v]
+        else:
+            blk["xyz".index(a[-1])] = v
+    return tuple(blk)

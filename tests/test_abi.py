@@ -124,6 +124,10 @@ def test_init_fails_loudly_without_device(lbm):
     (["-force", "maybe"], "on or off"),
     (["-bs", "100"], "multiple of 64"),
     (["-lambda", "-1"], "positive"),
+    (["-prec", "fp8"], "fp64, fp32s or fp16s"),
+    (["-prec"], "fp64, fp32s or fp16s"),
+    (["-fp16-scale", "3000"], "power of two"),
+    (["-fp16-scale", "0"], "must be in"),
 ])
 def test_kernel_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
@@ -131,7 +135,8 @@ def test_kernel_argument_errors(lbm, args, msg):
         lbm.Kernel("list-aa-pv-soa", geo, args)
 
 
-def test_aa_rejects_list_only_argument(lbm):
+@pytest.mark.parametrize("args", [["-pad", "auto"], ["-prec", "fp32s"], ["-fp16-scale", "1024"], ["-adj", "full"]])
+def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):
-        lbm.Kernel("aa-soa", geo, ["-pad", "auto"])
+        lbm.Kernel("aa-soa", geo, args)

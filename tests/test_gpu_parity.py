@@ -15,7 +15,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import golden, tiny_cases
+from conftest import blk_args as _blk_args, blk_tuple as _blk, golden, tiny_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -28,22 +28,6 @@ def _sha(a):
 
 def _relmax(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
-
-
-def _blk_args(g):
-    return [str(a) for a in g["kernel_args"] if str(a)]
-
-
-def _blk(args):
-    blk = [0, 0, 0]
-    it = iter(args)
-    for a in it:
-        v = int(next(it))
-        if a == "-blk":
-            blk = [v, v, v]
-        else:
-            blk["xyz".index(a[-1])] = v
-    return tuple(blk)
 
 
 def _run(lbm, name, geo, steps, omega, F, args=(), init="weights"):

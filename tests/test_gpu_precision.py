@@ -1,0 +1,169 @@
+"""GPU parity of the shifted-storage variants of list-aa-pv-soa (-prec fp32s /
+fp16s: deviations f - w stored in FP32 or as scaled binary16, collision in
+FP32), the FP64 moment reductions and the steady-state driver.
+
+Bars:
+  * every -prec variant against the oracle's shifted restatement
+    (orc_list_run_shifted), which follows the same operation order: node
+    views bit-exact;
+  * accuracy against the FP64 kernel on the 256^3 porous workload (the oracle
+    for the variants is pinned against the FP64 oracle on the CPU):
+    velocity relative L2 error and permeability relative error below the
+    tolerances in ACC_TOL;
+  * moment sums against sums of the node views, relative 1e-12 (mass) and
+    1e-10 (momentum, a difference of nearly equal terms).
+"""
+import numpy as np
+import pytest
+
+from conftest import blk_args, blk_tuple, golden, tiny_cases
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp32s", "fp16s"]
+LAM = 3 / 16
+# velocity relative L2 and permeability relative error vs FP64, 256^3 porous, 500 steps
+ACC_TOL = {"fp32s": (1e-4, 1e-5), "fp16s": (1e-2, 1e-3)}
+CX = np.array([0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0], dtype=float)
+
+
+def _gpu(lbm, geo, runs, omega, F, args, nodes=None):
+    with lbm.Kernel("list-aa-pv-soa", geo, list(args)) as k:
+        if nodes is None:
+            k.set_initial_weights()
+        else:
+            k.load(nodes)
+        for steps in runs:
+            k.run(lbm.case_data(omega=omega, xforce=F, iterations=steps))
+        return k.dump(), k.list_info()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("case", tiny_cases())
+def test_tiny_shifted_bitexact(lbm, oracle, case, prec):
+    g = golden(f"tiny_{case}.npz")
+    dims, per = tuple(int(v) for v in g["dims"]), tuple(int(v) for v in g["periodic"])
+    mask = g["mask"].astype(np.int32)
+    args = blk_args(g)
+    geo = lbm.Geometry.from_mask(mask, dims, per)
+    steps, omega, F = int(g["steps"]), float(g["omega"]), float(g["xforce"])
+    print(f"case {case} -prec {prec} dims={dims} args={args}", flush=True)
+    got, info = _gpu(lbm, geo, [steps], omega, F, args + ["-prec", prec, "-lambda", repr(LAM)])
+    assert info.prec == PRECS.index(prec) + 1
+    assert info.bytesPerFlupEven == {"fp32s": 152.0, "fp16s": 76.0}[prec]
+    want, _ = oracle.list_run_shifted(mask, dims, per, steps, omega, F, prec, magic=LAM, blk=blk_tuple(args))
+    assert np.array_equal(got, want), f"max diff {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_shifted_options_and_split_calls(lbm, oracle, prec):
+    geo = lbm.Geometry.from_string("porous-3-0.75-9", (48, 40, 36))
+    m = geo.mask.reshape(-1)
+    runs = [30, 20, 10]
+    want, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, runs, 1.25, 3e-5, prec, magic=LAM)
+    for extra in ([], ["-adj", "full"], ["-nt", "off"], ["-even-cap", "off", "-bs", "128"],
+                  ["-bs", "64", "-blk", "5"], ["-force", "on", "-adj", "full", "-nt", "off"]):
+        print(f"porous48 -prec {prec} {extra}", flush=True)
+        got, _ = _gpu(lbm, geo, runs, 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
+        assert np.array_equal(got, want), extra
+    got, _ = _gpu(lbm, geo, runs, 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM), "-force", "off"])
+    want_off, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, runs, 1.25, 3e-5, prec, magic=LAM,
+                                          force_on=False)
+    assert np.array_equal(got, want_off)
+
+
+def test_fp16_scale_option(lbm, oracle):
+    geo = lbm.Geometry.from_string("channel", (20, 14, 12), (1, 0, 0))
+    m = geo.mask.reshape(-1)
+    for scale, shift in (("1024", 10), ("65536", 16)):
+        got, _ = _gpu(lbm, geo, [80], 1.4, 1e-4, ["-prec", "fp16s", "-fp16-scale", scale])
+        want, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, 80, 1.4, 1e-4, "fp16s", f16_shift=shift)
+        assert np.array_equal(got, want), scale
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_shifted_setnode_getnode(lbm, oracle, prec):
+    geo = lbm.Geometry.from_string("porous-2-0.7-5", (20, 18, 16))
+    m = geo.mask.reshape(-1)
+    rng = np.random.default_rng(3)
+    nodes = np.tile(oracle.W, (geo.n_fluid, 1)) + 1e-3 * rng.standard_normal((geo.n_fluid, 19))
+    # encode/decode only: the oracle's conversion of the same node views
+    enc, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, [], 1.0, 0.0, prec, nodes=nodes)
+    with lbm.Kernel("list-aa-pv-soa", geo, ["-prec", prec]) as k:
+        k.load(nodes)
+        assert np.array_equal(k.dump(), enc)
+        k.run(lbm.case_data(iterations=0))          # upload and download unchanged
+        assert np.array_equal(k.dump(), enc)
+        k.run(lbm.case_data(omega=1.3, xforce=2e-5, iterations=40))
+        got = k.dump()
+    want, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, 40, 1.3, 2e-5, prec, nodes=nodes)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec", ["fp64"] + PRECS)
+def test_moments_match_node_sums(lbm, prec):
+    geo = lbm.Geometry.from_string("porous-3-0.75-9", (40, 36, 32))
+    with lbm.Kernel("list-aa-pv-soa", geo, ["-prec", prec, "-lambda", repr(LAM)]) as k:
+        k.set_initial_weights()
+        m0, j0 = k.moments()
+        assert m0 == pytest.approx(geo.n_fluid, rel=1e-14) and abs(j0) < 1e-12
+        k.run(lbm.case_data(omega=1.2, xforce=1e-4, iterations=60))
+        m, j = k.moments()
+        d = k.dump()
+    assert m == pytest.approx(d.sum(), rel=1e-12)
+    assert j == pytest.approx((d @ CX).sum(), rel=1e-10)
+    assert j > 0
+
+
+def test_run_steady(lbm, oracle):
+    geo = lbm.Geometry.from_string("channel", (14, 12, 10), (1, 0, 0))
+    m = geo.mask.reshape(-1)
+    cd = lbm.case_data(omega=1.1, xforce=1e-4, iterations=0)
+    # tol 0 never stops early: chunks of 16 steps (15 rounded up) up to 100
+    with lbm.Kernel("list-aa-pv-soa", geo) as k:
+        k.set_initial_weights()
+        r = k.run_steady(cd, 15, 0.0, 100)
+        assert (r.steps, r.converged) == (100, 0)
+        assert (r.mass, r.momentumX) == k.moments()
+        got = k.dump()
+    want, _ = oracle.list_run(m, geo.dims, geo.periodic, 100, 1.1, 1e-4, vsize=1)
+    assert np.array_equal(got, want)
+    # a loose tolerance stops at the first check that meets it
+    with lbm.Kernel("list-aa-pv-soa", geo, ["-prec", "fp32s"]) as k:
+        k.set_initial_weights()
+        r = k.run_steady(cd, 200, 1e-4, 40000)
+        assert r.converged == 1 and 0 <= r.relChange <= 1e-4 and r.steps % 200 == 0 and r.steps < 40000
+        assert r.seconds > 0 and k.duration == pytest.approx(r.seconds)
+
+
+def _ux_perm(d, n_cells, omega, F):
+    ux = d @ CX + F / 2                      # Luo forcing: physical velocity
+    nu = (1 / omega - 0.5) / 3
+    return ux, nu * ux.sum() / n_cells / F   # Darcy: k = nu * U_superficial / F
+
+
+def test_porous256_shifted_accuracy(lbm):
+    geo = lbm.Geometry.from_string("porous-6-0.8-42", (256, 256, 256))
+    cells = 256 ** 3
+    omega, F = 1.2, 1e-6
+    print("porous256 fp64 500 steps", flush=True)
+    ref, _ = _gpu(lbm, geo, [500], omega, F, ["-lambda", repr(LAM)])
+    u_ref, k_ref = _ux_perm(ref, cells, omega, F)
+    del ref
+    assert k_ref > 0
+    for prec in PRECS:
+        print(f"porous256 {prec} 500 steps", flush=True)
+        with lbm.Kernel("list-aa-pv-soa", geo, ["-prec", prec, "-lambda", repr(LAM)]) as k:
+            k.set_initial_weights()
+            k.run(lbm.case_data(omega=omega, xforce=F, iterations=500))
+            mass, _ = k.moments()
+            d = k.dump()
+        u, kp = _ux_perm(d, cells, omega, F)
+        del d
+        vel = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+        perm = abs(kp - k_ref) / k_ref
+        print(f"  {prec}: velocity rel L2 {vel:.3e}, permeability rel {perm:.3e}, mass rel "
+              f"{abs(mass - geo.n_fluid) / geo.n_fluid:.3e}", flush=True)
+        assert vel < ACC_TOL[prec][0]
+        assert perm < ACC_TOL[prec][1]
+        assert abs(mass - geo.n_fluid) / geo.n_fluid < 1e-6
