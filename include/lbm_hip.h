@@ -38,6 +38,10 @@
  *                     a Kernel() call, at the store).
  *   -fp16-scale <2^k> list only: power-of-two factor of the FP16 deviations
  *                     (default 4096)
+ *   -vec auto|1       list only, fp32s/fp16s: the even step takes 2 (FP32)
+ *                     or 4 (FP16) nodes per thread with 8-byte accesses when
+ *                     the padded node count allows (auto, default), or one
+ *                     node per thread (1)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H

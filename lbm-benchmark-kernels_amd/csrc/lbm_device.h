@@ -122,44 +122,55 @@ __device__ __forceinline__ void trt_collide(R (&f)[19], const TrtC<R> & k)
 namespace lbm {
 
 // Storage codecs of the PDF array.  load/store convert between the stored
-// value and the compute precision R; NT selects non-temporal accesses.
+// value and the compute precision R; NT selects non-temporal accesses.  NT
+// is a template argument so that each access is emitted with its hint (a
+// run-time choice between two loads of the same address gets merged into
+// one plain load).
+template <bool NT, class T>
+__device__ __forceinline__ T ld(const T * p)
+{
+	if constexpr (NT) return __builtin_nontemporal_load(p);
+	else return *p;
+}
+
+template <bool NT, class T>
+__device__ __forceinline__ void st(T * p, T v)
+{
+	if constexpr (NT) __builtin_nontemporal_store(v, p);
+	else *p = v;
+}
+
+// dec/enc convert one stored value; load/store add the access.
 struct CodecF64 {               // the reference's FP64 values
 	using S = double;
 	using R = double;
-	__device__ __forceinline__ R load(const S * p, bool nt) const { return nt ? __builtin_nontemporal_load(p) : *p; }
-	__device__ __forceinline__ void store(S * p, R v, bool nt) const
-	{
-		if (nt) __builtin_nontemporal_store(v, p);
-		else *p = v;
-	}
+	__device__ __forceinline__ R dec(S v) const { return v; }
+	__device__ __forceinline__ S enc(R v) const { return v; }
+	template <bool NT> __device__ __forceinline__ R load(const S * p) const { return ld<NT>(p); }
+	template <bool NT> __device__ __forceinline__ void store(S * p, R v) const { st<NT>(p, v); }
 };
 
 struct CodecF32 {               // deviation f - w in FP32
 	using S = float;
 	using R = float;
-	__device__ __forceinline__ R load(const S * p, bool nt) const { return nt ? __builtin_nontemporal_load(p) : *p; }
-	__device__ __forceinline__ void store(S * p, R v, bool nt) const
-	{
-		if (nt) __builtin_nontemporal_store(v, p);
-		else *p = v;
-	}
+	__device__ __forceinline__ R dec(S v) const { return v; }
+	__device__ __forceinline__ S enc(R v) const { return v; }
+	template <bool NT> __device__ __forceinline__ R load(const S * p) const { return ld<NT>(p); }
+	template <bool NT> __device__ __forceinline__ void store(S * p, R v) const { st<NT>(p, v); }
 };
 
 struct CodecF16 {               // deviation f - w times a power of two, in FP16
 	using S = uint16_t;
 	using R = float;
 	float scale, inv;            // 2^k and 2^-k: the scaling itself is exact
-	__device__ __forceinline__ R load(const S * p, bool nt) const
-	{
-		const uint16_t h = nt ? __builtin_nontemporal_load(p) : *p;
-		return __half2float(__ushort_as_half(h)) * inv;
-	}
-	__device__ __forceinline__ void store(S * p, R v, bool nt) const
-	{
-		const uint16_t h = __half_as_ushort(__float2half_rn(v * scale));
-		if (nt) __builtin_nontemporal_store(h, p);
-		else *p = h;
-	}
+	__device__ __forceinline__ R dec(S v) const { return __half2float(__ushort_as_half(v)) * inv; }
+	__device__ __forceinline__ S enc(R v) const { return (uint16_t)__half_as_ushort(__float2half_rn(v * scale)); }
+	template <bool NT> __device__ __forceinline__ R load(const S * p) const { return dec(ld<NT>(p)); }
+	template <bool NT> __device__ __forceinline__ void store(S * p, R v) const { st<NT>(p, enc(v)); }
 };
+
+// V consecutive stored values as one access.
+template <class S, int V>
+using VecS = S __attribute__((ext_vector_type(V)));
 
 }  // namespace lbm

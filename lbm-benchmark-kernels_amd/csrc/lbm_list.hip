@@ -321,12 +321,57 @@ __global__ __launch_bounds__(512) void k_list_even(typename Codec::S * pdf, Code
 	R f[19];
 #pragma unroll
 	for (int d = 0; d < 19; ++d) {
-		f[d] = cod.load(&pdf[(size_t)d * nCells + n], NT);
+		f[d] = cod.template load<NT>(&pdf[(size_t)d * nCells + n]);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	trt_collide(f, k);
 #pragma unroll
-	for (int d = 0; d < 19; ++d) cod.store(&pdf[(size_t)inv_dir(d) * nCells + n], f[d], NT);
+	for (int d = 0; d < 19; ++d) cod.template store<NT>(&pdf[(size_t)inv_dir(d) * nCells + n], f[d]);
+}
+
+// Even step over V consecutive nodes per thread, for the narrow storage
+// types: one 8-byte access per direction carries V values (V = 2 for FP32,
+// 4 for FP16), so a wavefront moves as many bytes per instruction as in
+// FP64.  Needs nCells % V == 0 (every plane starts V-aligned); slots past
+// nFluid in the last group are read (padding, zero-filled at Init) but not
+// written.  Each node's arithmetic is that of k_list_even.
+template <class Codec, int V, bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_list_even_v(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
+                                                     TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+{
+	using R = typename Codec::R;
+	using S = typename Codec::S;
+	using VT = VecS<S, V>;
+	const uint32_t n0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
+	if (n0 >= nFluid) return;
+	R f[V][19];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		const VT x = ld<NT>(reinterpret_cast<const VT *>(&pdf[(size_t)d * nCells + n0]));
+#pragma unroll
+		for (int j = 0; j < V; ++j) {
+			f[j][d] = cod.dec(x[j]);
+			if (FORCE_LOAD) f[j][d] = f[j][d] + ft.f[d];
+		}
+	}
+#pragma unroll
+	for (int j = 0; j < V; ++j) trt_collide(f[j], k);
+	if (n0 + V <= nFluid) {
+#pragma unroll
+		for (int d = 0; d < 19; ++d) {
+			VT y;
+#pragma unroll
+			for (int j = 0; j < V; ++j) y[j] = cod.enc(f[j][d]);
+			st<NT>(reinterpret_cast<VT *>(&pdf[(size_t)inv_dir(d) * nCells + n0]), y);
+		}
+	} else {
+		const uint32_t m = nFluid - n0;
+#pragma unroll
+		for (int d = 0; d < 19; ++d)
+#pragma unroll
+			for (int j = 0; j < V; ++j)
+				if ((uint32_t)j < m) cod.template store<NT>(&pdf[(size_t)inv_dir(d) * nCells + n0 + j], f[j][d]);
+	}
 }
 
 // Odd step: direction d arrives from slot adj[n][inv d] (the neighbour's
@@ -348,11 +393,11 @@ __global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec
 	R f[19];
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
-		f[d] = cod.load(&pdf[a[inv_dir(d)]], false);
+		f[d] = cod.template load<false>(&pdf[a[inv_dir(d)]]);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	const size_t cSlot = (size_t)dC * nCells + n;
-	f[dC] = cod.load(&pdf[cSlot], false);
+	f[dC] = cod.template load<false>(&pdf[cSlot]);
 	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
 	trt_collide(f, k);
 #pragma unroll
@@ -362,9 +407,9 @@ __global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec
 			const bool bounce = a[d] == (uint32_t)inv_dir(d) * nCells + n;
 			v = v + (bounce ? ft.f[inv_dir(d)] : ft.f[d]);
 		}
-		cod.store(&pdf[a[d]], v, false);
+		cod.template store<false>(&pdf[a[d]], v);
 	}
-	cod.store(&pdf[cSlot], FORCE_STORE ? f[dC] + ft.f[dC] : f[dC], false);
+	cod.template store<false>(&pdf[cSlot], FORCE_STORE ? f[dC] + ft.f[dC] : f[dC]);
 }
 
 // Node sums in the local layout (after a completed Kernel() call), in FP64:
@@ -380,7 +425,7 @@ __global__ __launch_bounds__(256) void k_list_moments(const typename Codec::S * 
 	double m = 0.0, jx = 0.0;
 	if (n < nFluid) {
 		for (int d = 0; d < 19; ++d) {
-			const double v = (double)cod.load(&pdf[(size_t)d * nCells + n], false);
+			const double v = (double)cod.template load<false>(&pdf[(size_t)d * nCells + n]);
 			m += v;
 			jx += cx(d) * v;
 		}
@@ -546,26 +591,50 @@ static double list_odd_bytes(const ListData * L)
 
 static double list_even_bytes(const ListData * L) { return 2.0 * 19 * (double)elem_bytes(L->c.args.prec); }
 
+// Nodes per thread of the even step: 8-byte accesses for the narrow storage
+// types when the plane stride allows aligned vectors (-vec 1 turns it off).
+static int list_even_vec(const ListData * L)
+{
+	const int v = L->c.args.prec == PREC_FP32S ? 2 : L->c.args.prec == PREC_FP16S ? 4 : 1;
+	if (!L->c.args.evenVec || L->nCells % v != 0) return 1;
+	return v;
+}
+
 struct LaunchCfg {
-	dim3 egrid, eblock;
+	dim3 eblock;
 	size_t elds;
 	dim3 grid, block;
 	hipStream_t stream;
 	uint32_t nF, nC;
 	bool rle, nt;
+	// blocks of the even step when each thread takes v nodes
+	unsigned evenThreads(int v) const
+	{
+		const uint32_t threads = (nF + (uint32_t)v - 1) / (uint32_t)v;
+		return (threads + eblock.x - 1) / eblock.x;
+	}
 };
 
-template <class Codec>
+template <class Codec, int V>
 static void launch_even(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, const TrtConsts & k0,
                         const ForceTerms & f0, bool fl)
 {
 	using R = typename Codec::R;
 	const TrtC<R> k = trt_cast<R>(k0);
 	const ForceT<R> ft = force_cast<R>(f0);
-	if (fl && c.nt)  k_list_even<Codec, true, true><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-	else if (fl)     k_list_even<Codec, true, false><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-	else if (c.nt)   k_list_even<Codec, false, true><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-	else             k_list_even<Codec, false, false><<<c.egrid, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	if constexpr (V == 1) {
+		const dim3 g(c.evenThreads(1));
+		if (fl && c.nt)  k_list_even<Codec, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else if (fl)     k_list_even<Codec, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else if (c.nt)   k_list_even<Codec, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else             k_list_even<Codec, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	} else {
+		const dim3 g(c.evenThreads(V));
+		if (fl && c.nt)  k_list_even_v<Codec, V, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else if (fl)     k_list_even_v<Codec, V, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else if (c.nt)   k_list_even_v<Codec, V, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		else             k_list_even_v<Codec, V, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+	}
 }
 
 template <class Codec, class Adj>
@@ -608,14 +677,21 @@ static bool list_steps(KernelData * kd, const CaseData * cd)
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
 	const StreamShape es = stream_shape(&c, bs);
-	const LaunchCfg cfg = {dim3((unsigned)((L->nFluid + es.block - 1) / es.block)), dim3((unsigned)es.block), es.lds,
-	                       grid, block, c.stream, nF, nC, c.args.adjRle != 0, c.args.nontemporal != 0};
+	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, c.stream, nF, nC,
+	                       c.args.adjRle != 0, c.args.nontemporal != 0};
 	const float f16s = (float)(1 << c.args.f16Shift);
+	const int vec = list_even_vec(L);
 	auto even = [&](bool fl) {
 		switch (c.args.prec) {
-			case PREC_FP32S: launch_even(cfg, (float *)L->dPdf, CodecF32{}, k, ft, fl); break;
-			case PREC_FP16S: launch_even(cfg, (uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, fl); break;
-			default:         launch_even(cfg, (double *)L->dPdf, CodecF64{}, k, ft, fl);
+			case PREC_FP32S:
+				if (vec == 2) launch_even<CodecF32, 2>(cfg, (float *)L->dPdf, CodecF32{}, k, ft, fl);
+				else          launch_even<CodecF32, 1>(cfg, (float *)L->dPdf, CodecF32{}, k, ft, fl);
+				break;
+			case PREC_FP16S:
+				if (vec == 4) launch_even<CodecF16, 4>(cfg, (uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, fl);
+				else          launch_even<CodecF16, 1>(cfg, (uint16_t *)L->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, fl);
+				break;
+			default: launch_even<CodecF64, 1>(cfg, (double *)L->dPdf, CodecF64{}, k, ft, fl);
 		}
 	};
 	auto odd = [&](bool fl, bool fs) {

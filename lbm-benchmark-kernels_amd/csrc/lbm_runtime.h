@@ -45,6 +45,7 @@ struct KernelArgs {
 	int evenCap = 1;       // even step: 512-thread blocks, one block per CU
 	int prec = 0;          // list: PDF storage, see enum Prec
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
+	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
 };
 
 // PDF storage of the list kernel.

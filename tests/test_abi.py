@@ -128,6 +128,7 @@ def test_init_fails_loudly_without_device(lbm):
     (["-prec"], "fp64, fp32s or fp16s"),
     (["-fp16-scale", "3000"], "power of two"),
     (["-fp16-scale", "0"], "must be in"),
+    (["-vec", "2"], "auto or 1"),
 ])
 def test_kernel_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
@@ -135,7 +136,8 @@ def test_kernel_argument_errors(lbm, args, msg):
         lbm.Kernel("list-aa-pv-soa", geo, args)
 
 
-@pytest.mark.parametrize("args", [["-pad", "auto"], ["-prec", "fp32s"], ["-fp16-scale", "1024"], ["-adj", "full"]])
+@pytest.mark.parametrize("args", [["-pad", "auto"], ["-prec", "fp32s"], ["-fp16-scale", "1024"], ["-adj", "full"],
+                                  ["-vec", "1"]])
 def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):
