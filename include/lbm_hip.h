@@ -150,6 +150,62 @@ typedef struct {
 } LbmHipSteady;
 int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out);
 
+/* ---- batch: many list-aa-pv-soa lattices stepped together --------------
+ * No counterpart in the reference (one lattice per process).  Every domain
+ * is an ordinary list-aa-pv-soa instance created with the batch's kernel
+ * arguments (LbmHipBatchKernelData gives its KernelData: GetNode, SetNode,
+ * LbmHipListMoments all apply), initialised to f = w (rho 1, u 0).  A step
+ * interval advances every active domain by the same number of steps in one
+ * even and one odd launch per step over a list of node chunks of all of
+ * them (replayed from a HIP graph) and then checks each domain's x momentum
+ * in FP64; a domain leaves the active set when the relative change between
+ * two checks is <= tol or its maxSteps (checked at interval ends) is
+ * reached.  Each domain's result is bit-identical to the same lattice run
+ * alone through Kernel() for the same number of steps (FP64 and fp32s;
+ * for fp16s, to Kernel() calls of one interval each).  After stepping, the
+ * host copies are stale until LbmHipBatchSync.  Not thread safe. */
+typedef struct LbmHipBatch LbmHipBatch;
+typedef struct {
+	int active;         /* 1 while the domain is stepped */
+	int converged;      /* 1 when it met the tolerance */
+	int steps;          /* steps run */
+	int nFluid;
+	double relChange;   /* last relative change of momentumX (-1 before two checks) */
+	double mass;        /* FP64 sums after the last interval */
+	double momentumX;
+	double seconds;     /* GPU time of the intervals it took part in */
+} LbmHipBatchStatus;
+typedef struct {
+	int evenBlock;      /* threads per block of the even step (64..512) */
+	int evenCap;        /* 1: one even-step block per compute unit (LDS reservation) */
+	int vec;            /* nodes per thread of the even step: 1, 2 (fp32s), 4 (fp16s) */
+	int oddBlock;       /* threads per block of the odd step (64..512) */
+	double evenMs, oddMs;   /* per launch, as measured by LbmHipBatchAutotune */
+} LbmHipBatchTuning;
+/* params: kernel arguments shared by all domains (-prec, -lambda, -adj, ...). */
+LbmHipBatch * LbmHipBatchCreate(const Parameters * params);
+void LbmHipBatchDestroy(LbmHipBatch * b);
+/* Adds a domain; returns its id (>= 0) or -1.  maxSteps <= 0: no limit. */
+int LbmHipBatchAdd(LbmHipBatch * b, LatticeDesc * ld, double omega, double xForce, int maxSteps);
+/* Frees a domain (active or not); its id may be reused by a later Add. */
+int LbmHipBatchRemove(LbmHipBatch * b, int id);
+KernelData * LbmHipBatchKernelData(LbmHipBatch * b, int id);
+int LbmHipBatchStatusOf(LbmHipBatch * b, int id, LbmHipBatchStatus * out);
+/* One interval of `steps` (rounded up to even) for all active domains;
+ * *nActive receives the number still active afterwards. */
+int LbmHipBatchStep(LbmHipBatch * b, int steps, double tol, int * nActive);
+/* Intervals of checkEvery steps until no domain is active. */
+int LbmHipBatchRun(LbmHipBatch * b, int checkEvery, double tol);
+/* Device -> host copy of one domain (id >= 0) or of all (id < 0). */
+int LbmHipBatchSync(LbmHipBatch * b, int id);
+int LbmHipBatchGetTuning(LbmHipBatch * b, LbmHipBatchTuning * out);
+int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
+/* Times `pairs` launches of every candidate shape on the active domains and
+ * keeps the fastest; the lattices are restored afterwards. */
+int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
+/* Summed GPU time of all intervals and fluid-node updates done. */
+int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);
+
 /* Copies the device list build to caller buffers, in the reference's
  * layouts (any pointer may be NULL):
  *   adj    nFluid*18 uint32, node-major (KernelDataList.AdjList)

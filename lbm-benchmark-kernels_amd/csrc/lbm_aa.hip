@@ -250,7 +250,7 @@ static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A, x, y, z)) return;
+	if (!aa_local_layout(A, x, y, z) || !shadow_writable(&A->c)) return;
 	for (int d = 0; d < 19; ++d) A->hPdf[aa_slot(A->g, d, x, y, z)] = pdfs[d];
 	A->c.shadowDirty = 1;
 }
@@ -265,7 +265,7 @@ static void aa_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
 static void aa_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 {
 	AaData * A = as_aa(kd);
-	if (!aa_local_layout(A, x, y, z)) return;
+	if (!aa_local_layout(A, x, y, z) || !shadow_writable(&A->c)) return;
 	A->hPdf[aa_slot(A->g, dir, x, y, z)] = pdf;
 	A->c.shadowDirty = 1;
 }
@@ -276,7 +276,9 @@ static bool aa_download(KernelData * kd)
 	AaData * A = as_aa(kd);
 	if (A->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
 	if (!hip_ok(hipMemcpyAsync(A->hPdf, A->dPdf, sizeof(double) * A->nPdf, hipMemcpyDeviceToHost, A->c.stream), "download of the device lattice")) return false;
-	return hip_ok(hipStreamSynchronize(A->c.stream), "hipStreamSynchronize");
+	if (!hip_ok(hipStreamSynchronize(A->c.stream), "hipStreamSynchronize")) return false;
+	A->c.shadowStale = 0;
+	return true;
 }
 
 static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
@@ -332,6 +334,7 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
+	c.shadowStale = A->nFluid > 0 && steps > 0;
 	if (c.shadowDownload) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
 		if (!aa_download(kd)) return;
@@ -381,9 +384,11 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	const uint64_t nTotal = (uint64_t)g.X * g.Y * g.Z;
 	if (nTotal > 0xffffffffull / 19) { fail("number of PDFs exceed 2^32."); aa_release(A); return; }
 
-	printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);
-	printf("# allocating data for %llu LB nodes with padding (%llu bytes = %f MiB for the single lattice)\n",
-	       (unsigned long long)(g.plane), (unsigned long long)(A->nPdf * 8), A->nPdf * 8.0 / 1024.0 / 1024.0);
+	if (!A->c.args.quiet) {
+		printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);
+		printf("# allocating data for %llu LB nodes with padding (%llu bytes = %f MiB for the single lattice)\n",
+		       (unsigned long long)(g.plane), (unsigned long long)(A->nPdf * 8), A->nPdf * 8.0 / 1024.0 / 1024.0);
+	}
 
 	int * dMask = nullptr;
 	uint32_t * dCount = nullptr;
@@ -412,7 +417,7 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	(void)hipFree(dCount);
 	if (!ok) { aa_release(A); return; }
 	A->nBounce = (int)hCount;
-	printf("# bounce back pdfs: %d (resolved in the odd step, no index lists)\n", A->nBounce);
+	if (!A->c.args.quiet) printf("# bounce back pdfs: %d (resolved in the odd step, no index lists)\n", A->nBounce);
 
 	KernelData * kd = &A->kd;
 	kd->Pdfs[0] = A->hPdf;

@@ -1,0 +1,829 @@
+// lbm_batch.hip -- many list-aa-pv-soa lattices stepped together on one GPU.
+//
+// No counterpart in the reference, which runs one lattice per process.  Each
+// lattice ("domain") is an ordinary list-aa-pv-soa kernel instance (its own
+// list build, adjacency, device PDFs and host shadow, so GetNode/SetNode and
+// every other entry point work on it); the batch adds:
+//
+//   * one step kernel per half-step for all active domains: a persistent
+//     grid walks a device-resident list of chunks (domain, first node), each
+//     chunk one block's worth of consecutive nodes of one domain, so small
+//     domains fill the GPU together and the per-domain constants (omega,
+//     force, pointers, sizes) come from a device table;
+//   * the step loop of one check interval captured once as a HIP graph and
+//     replayed; the chunk lists live in device memory, so domains that finish
+//     leave the graph valid (the host rewrites the lists between replays);
+//   * FP64 per-domain moments after every interval (segmented reduction in a
+//     fixed order) and a per-domain convergence test on the x momentum;
+//   * an autotuner over block sizes, even-step occupancy and nodes per lane.
+//
+// The per-node arithmetic is lbm_list.h's even_node / even_nodes / odd_node,
+// the same code the single-lattice kernels run, so a domain stepped in a
+// batch ends bit-identical to the same lattice stepped alone.
+#include "lbm_list.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+namespace lbm {
+
+template <class R>
+struct BatchDom {
+	void * pdf;
+	const unsigned long long * runMask;
+	const uint32_t * runBase;
+	const uint32_t * runAdj;
+	const uint32_t * adj;
+	uint32_t nFluid, nCells, runStride, adjStride;
+	TrtC<R> k;
+	ForceT<R> ft;
+};
+
+struct Chunk {
+	uint32_t dom;     // index into the domain table
+	uint32_t node0;   // first node (a multiple of the chunk size, hence of 64)
+};
+
+// ------------------------------------------------------------- kernels
+// Persistent grids: block b takes chunks b, b + gridDim.x, ... below *count.
+template <class Codec, int V, bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_batch_even(const BatchDom<typename Codec::R> * __restrict__ doms,
+                                                    const Chunk * __restrict__ chunks,
+                                                    const uint32_t * __restrict__ count, Codec cod)
+{
+	using R = typename Codec::R;
+	using S = typename Codec::S;
+	const uint32_t nc = *count;
+	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+		const Chunk ch = chunks[c];
+		const BatchDom<R> & D = doms[ch.dom];
+		const uint32_t n0 = ch.node0 + threadIdx.x * V;
+		if (n0 >= D.nFluid) continue;
+		if constexpr (V == 1) even_node<Codec, FORCE_LOAD, NT>((S *)D.pdf, cod, n0, D.nCells, D.k, D.ft);
+		else even_nodes<Codec, V, FORCE_LOAD, NT>((S *)D.pdf, cod, n0, D.nFluid, D.nCells, D.k, D.ft);
+	}
+}
+
+template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, bool RLE>
+__global__ __launch_bounds__(512) void k_batch_odd(const BatchDom<typename Codec::R> * __restrict__ doms,
+                                                   const Chunk * __restrict__ chunks,
+                                                   const uint32_t * __restrict__ count, Codec cod)
+{
+	using R = typename Codec::R;
+	using S = typename Codec::S;
+	const uint32_t nc = *count;
+	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+		const Chunk ch = chunks[c];
+		const BatchDom<R> & D = doms[ch.dom];
+		const uint32_t n = ch.node0 + threadIdx.x;
+		if (n >= D.nFluid) continue;
+		uint32_t a[18];
+		if constexpr (RLE) AdjRuns{D.runMask, D.runBase, D.runAdj, D.runStride}.fetch(n, a);
+		else AdjFull{D.adj, D.adjStride}.fetch(n, a);
+		odd_node<Codec, FORCE_LOAD, FORCE_STORE>((S *)D.pdf, cod, a, n, D.nCells, D.k, D.ft);
+	}
+}
+
+// Sum over a chunk of 256 nodes of sum_d v_d and sum_d c_x,d v_d (v the
+// stored value), in FP64.
+template <class Codec>
+__global__ __launch_bounds__(256) void k_batch_moments(const BatchDom<typename Codec::R> * __restrict__ doms,
+                                                       const Chunk * __restrict__ chunks, Codec cod,
+                                                       double * __restrict__ partial)
+{
+	using R = typename Codec::R;
+	using S = typename Codec::S;
+	const Chunk ch = chunks[blockIdx.x];
+	const BatchDom<R> & D = doms[ch.dom];
+	const uint32_t n = ch.node0 + threadIdx.x;
+	double m = 0.0, jx = 0.0;
+	if (n < D.nFluid) {
+		const S * pdf = (const S *)D.pdf;
+		for (int d = 0; d < 19; ++d) {
+			const double v = (double)cod.template load<false>(&pdf[(size_t)d * D.nCells + n]);
+			m += v;
+			jx += cx(d) * v;
+		}
+	}
+	__shared__ double sm[256], sj[256];
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		partial[2 * blockIdx.x] = sm[0];
+		partial[2 * blockIdx.x + 1] = sj[0];
+	}
+}
+
+// Block i adds the chunk sums of segment [seg[i], seg[i+1]) in a fixed order.
+__global__ __launch_bounds__(256) void k_batch_reduce(const uint32_t * __restrict__ seg, const double * __restrict__ partial,
+                                                      double * __restrict__ out)
+{
+	const uint32_t b = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+	double m = 0.0, jx = 0.0;
+	for (uint32_t i = b + threadIdx.x; i < e; i += 256) {
+		m += partial[2 * i];
+		jx += partial[2 * i + 1];
+	}
+	__shared__ double sm[256], sj[256];
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		out[2 * blockIdx.x] = sm[0];
+		out[2 * blockIdx.x + 1] = sj[0];
+	}
+}
+
+// ---------------------------------------------------------------- host
+struct BatchEntry {
+	KernelData * kd = nullptr;
+	ListData * L = nullptr;
+	int state = 0;              // 0 free slot, 1 active, 2 finished
+	double omega = 1.0, xForce = 0.0;
+	int maxSteps = 0;
+	LbmHipBatchStatus st = {};
+	bool havePrev = false;
+	double jxPrev = 0.0;
+};
+
+struct Batch {
+	std::vector<std::string> words;
+	std::vector<char *> argv;
+	Parameters params = {};
+	KernelArgs args;
+	int device = 0, nCU = 0, ldsPerCU = 0, ldsPerBlock = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	std::vector<BatchEntry> e;
+	LbmHipBatchTuning tune = {};
+
+	// device tables (sized on rebuild; reallocation invalidates the graph)
+	void * dDoms = nullptr;
+	size_t domsBytes = 0;
+	Chunk * dEven = nullptr, * dOdd = nullptr, * dMom = nullptr;
+	size_t evenCapN = 0, oddCapN = 0, momCapN = 0;
+	uint32_t * dCount = nullptr;  // [0] even chunks, [1] odd chunks
+	uint32_t * dSeg = nullptr;
+	size_t segCapN = 0;
+	double * dPartial = nullptr, * dSums = nullptr;
+	size_t sumCapN = 0;
+	std::vector<Chunk> hEven, hOdd, hMom;
+	std::vector<uint32_t> hSeg;
+	std::vector<int> activeIds;    // domain id of moment segment i
+	std::vector<double> hSums;
+	bool tablesDirty = true;
+	uint32_t evenGrid = 1, oddGrid = 1;
+	size_t evenLds = 0;
+
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	int graphSteps = 0;
+	bool graphDirty = true;
+
+	double gpuMs = 0.0, flups = 0.0;
+};
+
+static Batch * as_batch(LbmHipBatch * b) { return reinterpret_cast<Batch *>(b); }
+
+static float f16_scale(const Batch * b) { return (float)(1 << b->args.f16Shift); }
+
+static int vec_max(int prec) { return prec == PREC_FP32S ? 2 : prec == PREC_FP16S ? 4 : 1; }
+
+// Nodes per thread of the batch's even step: the tuned value when every
+// active domain's plane stride allows aligned vectors of it, else 1.
+static int batch_vec(const Batch * b)
+{
+	int v = b->tune.vec > 0 ? b->tune.vec : 1;
+	if (v > vec_max(b->args.prec)) v = 1;
+	for (const BatchEntry & x : b->e)
+		if (x.state == 1 && x.L->nCells % v) return 1;
+	return v;
+}
+
+static bool grow(void ** p, size_t * capN, size_t needN, size_t elem, const char * what, bool * moved)
+{
+	if (needN <= *capN && *p) return true;
+	(void)hipFree(*p);
+	*p = nullptr;
+	const size_t n = needN + needN / 4 + 64;
+	const hipError_t err = hipMalloc(p, n * elem);
+	if (err != hipSuccess) { *capN = 0; return fail("allocating %s failed: %zu bytes (%s).", what, n * elem, hipGetErrorString(err)); }
+	*capN = n;
+	*moved = true;
+	return true;
+}
+
+template <class R>
+static void fill_doms(Batch * b, std::vector<unsigned char> & out)
+{
+	out.assign(sizeof(BatchDom<R>) * b->e.size(), 0);
+	BatchDom<R> * t = reinterpret_cast<BatchDom<R> *>(out.data());
+	for (size_t i = 0; i < b->e.size(); ++i) {
+		const BatchEntry & x = b->e[i];
+		if (!x.L) continue;
+		const ListData * L = x.L;
+		BatchDom<R> & D = t[i];
+		D.pdf = L->dPdf;
+		D.runMask = L->dRunMask;
+		D.runBase = L->dRunBase;
+		D.runAdj = L->dRunAdj;
+		D.adj = L->dAdj;
+		D.nFluid = (uint32_t)L->nFluid;
+		D.nCells = (uint32_t)L->nCells;
+		D.runStride = (uint32_t)L->runStride;
+		D.adjStride = (uint32_t)L->adjStride;
+		D.k = trt_cast<R>(trt_consts(x.omega, b->args.magic));
+		D.ft = force_cast<R>(force_terms(x.xForce));
+	}
+}
+
+// Step kernel of V nodes per thread, or of one when V exceeds the codec's.
+template <class Codec, int V>
+static const void * even_fn()
+{
+	if constexpr (V <= Codec::VMAX) return (const void *)k_batch_even<Codec, V, true, true>;
+	else return (const void *)k_batch_even<Codec, 1, true, true>;
+}
+
+template <class Codec>
+static bool even_occupancy(int v, int block, size_t lds, int * perCU)
+{
+	const void * fn = v == 4 ? even_fn<Codec, 4>() : v == 2 ? even_fn<Codec, 2>() : even_fn<Codec, 1>();
+	return hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(perCU, fn, block, lds), "occupancy query");
+}
+
+// Chunk lists of the active domains, the domain table and the moment
+// segments; grid sizes of the persistent launches.
+static bool rebuild_tables(Batch * b)
+{
+	const int v = batch_vec(b);
+	const uint32_t eSize = (uint32_t)(b->tune.evenBlock * v), oSize = (uint32_t)b->tune.oddBlock;
+	b->hEven.clear();
+	b->hOdd.clear();
+	b->hMom.clear();
+	b->hSeg.assign(1, 0);
+	b->activeIds.clear();
+	for (size_t i = 0; i < b->e.size(); ++i) {
+		const BatchEntry & x = b->e[i];
+		if (x.state != 1) continue;
+		const uint32_t nF = (uint32_t)x.L->nFluid;
+		for (uint32_t n = 0; n < nF; n += eSize) b->hEven.push_back({(uint32_t)i, n});
+		for (uint32_t n = 0; n < nF; n += oSize) b->hOdd.push_back({(uint32_t)i, n});
+		for (uint32_t n = 0; n < nF; n += 256) b->hMom.push_back({(uint32_t)i, n});
+		b->hSeg.push_back((uint32_t)b->hMom.size());
+		b->activeIds.push_back((int)i);
+	}
+	bool moved = false;
+	std::vector<unsigned char> doms;
+	if (b->args.prec == PREC_FP64) fill_doms<double>(b, doms);
+	else fill_doms<float>(b, doms);
+	size_t domsCapN = b->domsBytes;
+	if (!grow(&b->dDoms, &domsCapN, doms.size(), 1, "domain table", &moved)) return false;
+	b->domsBytes = domsCapN;
+	if (!grow((void **)&b->dEven, &b->evenCapN, b->hEven.size(), sizeof(Chunk), "even chunk list", &moved)) return false;
+	if (!grow((void **)&b->dOdd, &b->oddCapN, b->hOdd.size(), sizeof(Chunk), "odd chunk list", &moved)) return false;
+	if (!grow((void **)&b->dMom, &b->momCapN, b->hMom.size(), sizeof(Chunk), "moment chunk list", &moved)) return false;
+	if (!grow((void **)&b->dSeg, &b->segCapN, b->hSeg.size(), sizeof(uint32_t), "moment segments", &moved)) return false;
+	const size_t sumsN = 2 * (b->hMom.size() > b->activeIds.size() ? b->hMom.size() : b->activeIds.size());
+	size_t cap = b->sumCapN;
+	if (!grow((void **)&b->dPartial, &cap, sumsN, sizeof(double), "moment sums", &moved)) return false;
+	cap = b->sumCapN;
+	if (!grow((void **)&b->dSums, &cap, sumsN, sizeof(double), "moment sums", &moved)) return false;
+	b->sumCapN = cap;
+	if (!b->dCount && !hip_ok(hipMalloc((void **)&b->dCount, 2 * sizeof(uint32_t)), "allocating chunk counts")) return false;
+	if (moved) b->graphDirty = true;
+
+	const uint32_t counts[2] = {(uint32_t)b->hEven.size(), (uint32_t)b->hOdd.size()};
+	hipStream_t s = b->stream;
+	if (!doms.empty() && !hip_ok(hipMemcpyAsync(b->dDoms, doms.data(), doms.size(), hipMemcpyHostToDevice, s), "table upload")) return false;
+	if (!b->hEven.empty() && !hip_ok(hipMemcpyAsync(b->dEven, b->hEven.data(), sizeof(Chunk) * b->hEven.size(), hipMemcpyHostToDevice, s), "chunk upload")) return false;
+	if (!b->hOdd.empty() && !hip_ok(hipMemcpyAsync(b->dOdd, b->hOdd.data(), sizeof(Chunk) * b->hOdd.size(), hipMemcpyHostToDevice, s), "chunk upload")) return false;
+	if (!b->hMom.empty() && !hip_ok(hipMemcpyAsync(b->dMom, b->hMom.data(), sizeof(Chunk) * b->hMom.size(), hipMemcpyHostToDevice, s), "chunk upload")) return false;
+	if (!hip_ok(hipMemcpyAsync(b->dSeg, b->hSeg.data(), sizeof(uint32_t) * b->hSeg.size(), hipMemcpyHostToDevice, s), "segment upload")) return false;
+	if (!hip_ok(hipMemcpyAsync(b->dCount, counts, sizeof(counts), hipMemcpyHostToDevice, s), "count upload")) return false;
+	if (!hip_ok(hipStreamSynchronize(s), "table upload")) return false;
+
+	// persistent grids: the blocks that fit on the device at once
+	// the single-lattice even step's occupancy cap (stream_shape)
+	const size_t want = (size_t)b->ldsPerCU / 2 + 8192;
+	b->evenLds = (b->tune.evenCap && b->ldsPerCU > 0 && want <= (size_t)b->ldsPerBlock) ? want : 0;
+	int perCU = 1;
+	const int blk = b->tune.evenBlock;
+	bool ok;
+	switch (b->args.prec) {
+		case PREC_FP32S: ok = even_occupancy<CodecF32>(v, blk, b->evenLds, &perCU); break;
+		case PREC_FP16S: ok = even_occupancy<CodecF16>(v, blk, b->evenLds, &perCU); break;
+		default: ok = even_occupancy<CodecF64>(1, blk, b->evenLds, &perCU);
+	}
+	if (!ok) return false;
+	if (perCU < 1) perCU = 1;
+	const uint32_t eg = (uint32_t)(perCU * b->nCU), og0 = (uint32_t)((2048 / b->tune.oddBlock) * b->nCU);
+	int oddPerCU = 1;
+	const void * oddFn = b->args.prec == PREC_FP64 ? (const void *)k_batch_odd<CodecF64, true, true, true>
+	                   : b->args.prec == PREC_FP32S ? (const void *)k_batch_odd<CodecF32, true, true, true>
+	                   : (const void *)k_batch_odd<CodecF16, true, true, true>;
+	if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&oddPerCU, oddFn, b->tune.oddBlock, 0), "occupancy query")) return false;
+	const uint32_t og = oddPerCU > 0 ? (uint32_t)(oddPerCU * b->nCU) : og0;
+	const uint32_t ne = counts[0] ? counts[0] : 1, no = counts[1] ? counts[1] : 1;
+	const uint32_t newEven = eg < ne ? eg : ne, newOdd = og < no ? og : no;
+	// the graph keeps its grid: a shrinking chunk list needs no new capture
+	if (newEven > b->evenGrid || newOdd > b->oddGrid) b->graphDirty = true;
+	if (b->graphDirty) {
+		b->evenGrid = newEven;
+		b->oddGrid = newOdd;
+	}
+	b->tablesDirty = false;
+	return true;
+}
+
+template <class Codec, int V>
+static void launch_batch_even(Batch * b, Codec cod, bool fl)
+{
+	static_assert(V <= Codec::VMAX, "more nodes per access than the storage type packs");
+	using R = typename Codec::R;
+	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
+	const dim3 g(b->evenGrid), t(b->tune.evenBlock);
+	const bool nt = b->args.nontemporal != 0;
+	if (fl && nt)  k_batch_even<Codec, V, true, true><<<g, t, b->evenLds, b->stream>>>(doms, b->dEven, b->dCount, cod);
+	else if (fl)   k_batch_even<Codec, V, true, false><<<g, t, b->evenLds, b->stream>>>(doms, b->dEven, b->dCount, cod);
+	else if (nt)   k_batch_even<Codec, V, false, true><<<g, t, b->evenLds, b->stream>>>(doms, b->dEven, b->dCount, cod);
+	else           k_batch_even<Codec, V, false, false><<<g, t, b->evenLds, b->stream>>>(doms, b->dEven, b->dCount, cod);
+}
+
+template <class Codec, bool RLE>
+static void launch_batch_odd(Batch * b, Codec cod, bool fl, bool fs)
+{
+	using R = typename Codec::R;
+	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
+	const dim3 g(b->oddGrid), t(b->tune.oddBlock);
+	if (fs)      k_batch_odd<Codec, true, true, RLE><<<g, t, 0, b->stream>>>(doms, b->dOdd, b->dCount + 1, cod);
+	else if (fl) k_batch_odd<Codec, true, false, RLE><<<g, t, 0, b->stream>>>(doms, b->dOdd, b->dCount + 1, cod);
+	else         k_batch_odd<Codec, false, false, RLE><<<g, t, 0, b->stream>>>(doms, b->dOdd, b->dCount + 1, cod);
+}
+
+template <class Codec>
+static void launch_even_v(Batch * b, Codec cod, int v, bool fl)
+{
+	if constexpr (Codec::VMAX >= 4) { if (v == 4) { launch_batch_even<Codec, 4>(b, cod, fl); return; } }
+	if constexpr (Codec::VMAX >= 2) { if (v == 2) { launch_batch_even<Codec, 2>(b, cod, fl); return; } }
+	launch_batch_even<Codec, 1>(b, cod, fl);
+}
+
+// One even and one odd launch for every active domain.
+template <class Codec>
+static void launch_pair(Batch * b, Codec cod, int v, bool force, bool first, bool last)
+{
+	launch_even_v(b, cod, v, force && !first);
+	if (b->args.adjRle) launch_batch_odd<Codec, true>(b, cod, force, force && last);
+	else launch_batch_odd<Codec, false>(b, cod, force, force && last);
+}
+
+static void launch_steps(Batch * b, int steps)
+{
+	const int v = batch_vec(b);
+	const bool force = b->args.forceOn != 0;
+	const float s16 = f16_scale(b);
+	for (int it = 0; it < steps; it += 2) {
+		const bool first = it == 0, last = it + 2 >= steps;
+		switch (b->args.prec) {
+			case PREC_FP32S: launch_pair(b, CodecF32{}, v, force, first, last); break;
+			case PREC_FP16S: launch_pair(b, CodecF16{s16, 1.0f / s16}, v, force, first, last); break;
+			default: launch_pair(b, CodecF64{}, v, force, first, last);
+		}
+	}
+}
+
+static void drop_graph(Batch * b)
+{
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+	b->graphSteps = 0;
+}
+
+// The launches of `steps` steps recorded as one graph (kernel nodes only).
+static bool capture(Batch * b, int steps)
+{
+	drop_graph(b);
+	(void)hipGetLastError();
+	if (!hip_ok(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
+	launch_steps(b, steps);
+	const hipError_t launchErr = hipGetLastError();
+	const hipError_t endErr = hipStreamEndCapture(b->stream, &b->graph);
+	if (!hip_ok(launchErr, "batch step launch") || !hip_ok(endErr, "hipStreamEndCapture")) { drop_graph(b); return false; }
+	if (!hip_ok(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate")) { drop_graph(b); return false; }
+	b->graphSteps = steps;
+	b->graphDirty = false;
+	return true;
+}
+
+template <class Codec>
+static void launch_moments(Batch * b, Codec cod)
+{
+	using R = typename Codec::R;
+	k_batch_moments<Codec><<<(unsigned)b->hMom.size(), 256, 0, b->stream>>>((const BatchDom<R> *)b->dDoms, b->dMom, cod, b->dPartial);
+}
+
+// FP64 mass and x momentum of every active domain into its status.
+static bool batch_moments(Batch * b)
+{
+	const size_t nA = b->activeIds.size();
+	if (nA == 0) return true;
+	const float s16 = f16_scale(b);
+	(void)hipGetLastError();
+	if (!b->hMom.empty()) {
+		switch (b->args.prec) {
+			case PREC_FP32S: launch_moments(b, CodecF32{}); break;
+			case PREC_FP16S: launch_moments(b, CodecF16{s16, 1.0f / s16}); break;
+			default: launch_moments(b, CodecF64{});
+		}
+	}
+	k_batch_reduce<<<(unsigned)nA, 256, 0, b->stream>>>(b->dSeg, b->dPartial, b->dSums);
+	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
+	b->hSums.resize(2 * nA);
+	if (!hip_ok(hipMemcpyAsync(b->hSums.data(), b->dSums, sizeof(double) * 2 * nA, hipMemcpyDeviceToHost, b->stream), "moment download")) return false;
+	if (!hip_ok(hipStreamSynchronize(b->stream), "moment kernels")) return false;
+	for (size_t i = 0; i < nA; ++i) {
+		BatchEntry & x = b->e[b->activeIds[i]];
+		x.st.mass = b->hSums[2 * i] + (b->args.prec == PREC_FP64 ? 0.0 : (double)x.L->nFluid);
+		x.st.momentumX = b->hSums[2 * i + 1];
+	}
+	return true;
+}
+
+static bool batch_step(Batch * b, int steps, double tol)
+{
+	steps += steps & 1;
+	bool any = false;
+	for (BatchEntry & x : b->e) {
+		if (x.state != 1) continue;
+		any = true;
+		if (!list_upload(x.L)) return false;       // SetNode changes since the last interval
+	}
+	if (!any || steps <= 0) return true;
+	if (b->tablesDirty && !rebuild_tables(b)) return false;
+	if (b->graphDirty || b->graphSteps != steps) {
+		if (!capture(b, steps)) return false;
+	}
+	(void)hipGetLastError();
+	if (!hip_ok(hipEventRecord(b->ev[0], b->stream), "hipEventRecord")) return false;
+	if (!hip_ok(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch")) return false;
+	if (!hip_ok(hipEventRecord(b->ev[1], b->stream), "hipEventRecord")) return false;
+	if (!batch_moments(b)) return false;
+	float ms = 0.0f;
+	if (!hip_ok(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "hipEventElapsedTime")) return false;
+	b->gpuMs += ms;
+	for (int id : b->activeIds) {
+		BatchEntry & x = b->e[id];
+		b->flups += (double)x.L->nFluid * steps;
+		x.st.steps += steps;
+		x.st.seconds += ms / 1e3;
+		x.L->c.iteration = x.st.steps - 1;
+		x.L->c.shadowStale = 1;
+		const double jx = x.st.momentumX;
+		if (x.havePrev) {
+			const double d = fabs(jx - x.jxPrev), a = fabs(jx);
+			x.st.relChange = a > 0.0 ? d / a : (d > 0.0 ? INFINITY : 0.0);
+			if (x.st.relChange <= tol) x.st.converged = 1;
+		}
+		x.jxPrev = jx;
+		x.havePrev = true;
+		if (x.st.converged || (x.maxSteps > 0 && x.st.steps >= x.maxSteps)) {
+			x.state = 2;
+			x.st.active = 0;
+			b->tablesDirty = true;
+		}
+	}
+	return true;
+}
+
+static void batch_free_tables(Batch * b)
+{
+	drop_graph(b);
+	(void)hipFree(b->dDoms);
+	(void)hipFree(b->dEven);
+	(void)hipFree(b->dOdd);
+	(void)hipFree(b->dMom);
+	(void)hipFree(b->dCount);
+	(void)hipFree(b->dSeg);
+	(void)hipFree(b->dPartial);
+	(void)hipFree(b->dSums);
+}
+
+static BatchEntry * entry(Batch * b, int id, const char * who)
+{
+	if (!b || id < 0 || id >= (int)b->e.size() || b->e[id].state == 0) {
+		fail("%s: no domain %d in this batch.", who, id);
+		return nullptr;
+	}
+	return &b->e[id];
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+LbmHipBatch * LbmHipBatchCreate(const Parameters * params)
+{
+	Batch * b = new (std::nothrow) Batch();
+	if (!b) { fail("allocating the batch failed."); return nullptr; }
+	const int n = params ? params->nKernelArgs : 0;
+	for (int i = 0; i < n; ++i) b->words.emplace_back(params->KernelArgs[i]);
+	b->words.emplace_back("-quiet");
+	for (std::string & w : b->words) b->argv.push_back(&w[0]);
+	b->params.nKernelArgs = (int)b->argv.size();
+	b->params.KernelArgs = b->argv.data();
+	bool ok = parse_kernel_args(&b->params, &b->args, true);
+	int nDev = 0;
+	if (ok) {
+		const hipError_t err = hipGetDeviceCount(&nDev);
+		if (err == hipErrorNoDevice || (err == hipSuccess && nDev < 1))
+			ok = fail("no HIP device found; the list/aa kernels of this library run on the GPU only.");
+		else ok = hip_ok(err, "hipGetDeviceCount");
+	}
+	ok = ok && hip_ok(hipGetDevice(&b->device), "hipGetDevice")
+	        && hip_ok(hipDeviceGetAttribute(&b->nCU, hipDeviceAttributeMultiprocessorCount, b->device), "hipDeviceGetAttribute")
+	        && hip_ok(hipDeviceGetAttribute(&b->ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, b->device),
+	                  "hipDeviceGetAttribute")
+	        && hip_ok(hipDeviceGetAttribute(&b->ldsPerBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device),
+	                  "hipDeviceGetAttribute")
+	        && hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate")
+	        && hip_ok(hipEventCreate(&b->ev[0]), "hipEventCreate") && hip_ok(hipEventCreate(&b->ev[1]), "hipEventCreate");
+	if (!ok) {
+		LbmHipBatchDestroy(reinterpret_cast<LbmHipBatch *>(b));
+		return nullptr;
+	}
+	// defaults: the single-lattice launch shapes
+	b->tune.evenBlock = b->args.evenCap ? 512 : b->args.blockSize;
+	b->tune.evenCap = b->args.evenCap;
+	b->tune.vec = b->args.evenVec ? vec_max(b->args.prec) : 1;
+	b->tune.oddBlock = b->args.blockSize;
+	return reinterpret_cast<LbmHipBatch *>(b);
+}
+
+void LbmHipBatchDestroy(LbmHipBatch * handle)
+{
+	Batch * b = as_batch(handle);
+	if (!b) return;
+	for (BatchEntry & x : b->e)
+		if (x.kd) D3Q19ListAaPvDeinit_PushSoAHip(nullptr, &x.kd);
+	batch_free_tables(b);
+	if (b->ev[0]) (void)hipEventDestroy(b->ev[0]);
+	if (b->ev[1]) (void)hipEventDestroy(b->ev[1]);
+	if (b->stream) (void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+int LbmHipBatchAdd(LbmHipBatch * handle, LatticeDesc * ld, double omega, double xForce, int maxSteps)
+{
+	Batch * b = as_batch(handle);
+	if (!b || !ld) { fail("LbmHipBatchAdd: invalid arguments."); return -1; }
+	if (!(omega > 0.0 && omega < 2.0)) { fail("omega must be in (0, 2), got %f.", omega); return -1; }
+	KernelData * kd = nullptr;
+	D3Q19ListAaPvInit_PushSoAHip(ld, &kd, &b->params);
+	if (!kd) return -1;
+	ListData * L = list_data(kd);
+	// f = w (rho 1, u 0): zero bits for shifted storage, the weights for FP64
+	if (b->args.prec == PREC_FP64) {
+		double * h = (double *)L->hPdf;
+		for (int d = 0; d < 19; ++d)
+			for (int n = 0; n < L->nFluid; ++n) h[(size_t)d * L->nCells + n] = kW[d];
+		L->c.shadowDirty = 1;
+	}
+	int id = -1;
+	for (size_t i = 0; i < b->e.size(); ++i)
+		if (b->e[i].state == 0) { id = (int)i; break; }
+	if (id < 0) { id = (int)b->e.size(); b->e.emplace_back(); }
+	BatchEntry & x = b->e[id];
+	x = BatchEntry();
+	x.kd = kd;
+	x.L = L;
+	x.state = 1;
+	x.omega = omega;
+	x.xForce = xForce;
+	x.maxSteps = maxSteps;
+	x.st.active = 1;
+	x.st.relChange = -1.0;
+	x.st.nFluid = L->nFluid;
+	b->tablesDirty = true;
+	return id;
+}
+
+int LbmHipBatchRemove(LbmHipBatch * handle, int id)
+{
+	Batch * b = as_batch(handle);
+	BatchEntry * x = entry(b, id, "LbmHipBatchRemove");
+	if (!x) return -1;
+	D3Q19ListAaPvDeinit_PushSoAHip(nullptr, &x->kd);
+	*x = BatchEntry();
+	b->tablesDirty = true;
+	return 0;
+}
+
+KernelData * LbmHipBatchKernelData(LbmHipBatch * handle, int id)
+{
+	BatchEntry * x = entry(as_batch(handle), id, "LbmHipBatchKernelData");
+	return x ? x->kd : nullptr;
+}
+
+int LbmHipBatchStatusOf(LbmHipBatch * handle, int id, LbmHipBatchStatus * out)
+{
+	BatchEntry * x = entry(as_batch(handle), id, "LbmHipBatchStatusOf");
+	if (!x || !out) return -1;
+	*out = x->st;
+	return 0;
+}
+
+int LbmHipBatchStep(LbmHipBatch * handle, int steps, double tol, int * nActive)
+{
+	Batch * b = as_batch(handle);
+	if (!b || steps < 0 || !(tol >= 0.0)) { fail("LbmHipBatchStep: invalid arguments."); return -1; }
+	if (!batch_step(b, steps, tol)) return -1;
+	if (nActive) {
+		int k = 0;
+		for (const BatchEntry & x : b->e) k += x.state == 1;
+		*nActive = k;
+	}
+	return 0;
+}
+
+int LbmHipBatchRun(LbmHipBatch * handle, int checkEvery, double tol)
+{
+	Batch * b = as_batch(handle);
+	if (!b || checkEvery <= 0) { fail("LbmHipBatchRun: invalid arguments."); return -1; }
+	for (const BatchEntry & x : b->e)
+		if (x.state == 1 && x.maxSteps <= 0 && !(tol > 0.0)) {
+			fail("LbmHipBatchRun: a domain without maxSteps needs tol > 0 to end.");
+			return -1;
+		}
+	int active = 1;
+	while (active > 0)
+		if (LbmHipBatchStep(handle, checkEvery, tol, &active) != 0) return -1;
+	return 0;
+}
+
+int LbmHipBatchSync(LbmHipBatch * handle, int id)
+{
+	Batch * b = as_batch(handle);
+	if (id >= 0) {
+		BatchEntry * x = entry(b, id, "LbmHipBatchSync");
+		return (x && (!x->L->c.shadowStale || list_download(x->L))) ? 0 : -1;
+	}
+	if (!b) { fail("LbmHipBatchSync: invalid batch."); return -1; }
+	for (BatchEntry & x : b->e)
+		if (x.state != 0 && x.L->c.shadowStale && !list_download(x.L)) return -1;
+	return 0;
+}
+
+int LbmHipBatchGetTuning(LbmHipBatch * handle, LbmHipBatchTuning * out)
+{
+	Batch * b = as_batch(handle);
+	if (!b || !out) { fail("LbmHipBatchGetTuning: invalid arguments."); return -1; }
+	*out = b->tune;
+	out->vec = batch_vec(b);
+	return 0;
+}
+
+int LbmHipBatchSetTuning(LbmHipBatch * handle, const LbmHipBatchTuning * t)
+{
+	Batch * b = as_batch(handle);
+	if (!b || !t) { fail("LbmHipBatchSetTuning: invalid arguments."); return -1; }
+	const bool okBlock = t->evenBlock >= 64 && t->evenBlock <= 512 && t->evenBlock % 64 == 0 &&
+	                     t->oddBlock >= 64 && t->oddBlock <= 512 && t->oddBlock % 64 == 0;
+	if (!okBlock || (t->vec != 1 && t->vec != 2 && t->vec != 4) || t->vec > vec_max(b->args.prec)) {
+		fail("LbmHipBatchSetTuning: blocks must be 64..512 in steps of 64, vec 1, 2 (fp32s) or 4 (fp16s).");
+		return -1;
+	}
+	b->tune = *t;
+	b->tune.evenCap = t->evenCap != 0;
+	b->tablesDirty = true;
+	b->graphDirty = true;
+	return 0;
+}
+
+int LbmHipBatchStats(LbmHipBatch * handle, double * gpuSeconds, double * flups)
+{
+	Batch * b = as_batch(handle);
+	if (!b) { fail("LbmHipBatchStats: invalid batch."); return -1; }
+	if (gpuSeconds) *gpuSeconds = b->gpuMs / 1e3;
+	if (flups) *flups = b->flups;
+	return 0;
+}
+
+// Times `pairs` step pairs of every candidate launch shape on the active
+// domains and keeps the fastest even and odd shapes.  The lattices are
+// stepped while tuning, so they are restored from their host copies
+// afterwards (downloaded first where the device was newer).
+int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out)
+{
+	Batch * b = as_batch(handle);
+	if (!b || pairs <= 0) { fail("LbmHipBatchAutotune: invalid arguments."); return -1; }
+	for (BatchEntry & x : b->e) {
+		if (x.state != 1) continue;
+		if (x.L->c.shadowDirty) { if (!list_upload(x.L)) return -1; }
+		else if (x.L->c.shadowStale && !list_download(x.L)) return -1;
+	}
+	const LbmHipBatchTuning saved = b->tune;
+	const int vm = b->args.evenVec ? vec_max(b->args.prec) : 1;
+	struct Cand { int block, cap, vec; };
+	std::vector<Cand> evens;
+	for (int cap = 1; cap >= 0; --cap)
+		for (int blk : {256, 512})
+			for (int v = vm; v >= 1; v /= 2) evens.push_back({blk, cap, v});
+	const int odds[] = {128, 256, 512};
+	float bestEven = 1e30f, bestOdd = 1e30f;
+	LbmHipBatchTuning best = saved;
+	bool ok = true;
+	hipEvent_t t0 = b->ev[0], t1 = b->ev[1];
+	auto time_it = [&](bool evenOnly, float * ms) -> bool {
+		if (b->tablesDirty && !rebuild_tables(b)) return false;
+		(void)hipGetLastError();
+		const float s16 = f16_scale(b);
+		const int v = batch_vec(b);
+		if (!hip_ok(hipEventRecord(t0, b->stream), "hipEventRecord")) return false;
+		for (int p = 0; p < pairs; ++p) {
+			if (evenOnly) {
+				switch (b->args.prec) {
+					case PREC_FP32S: launch_even_v(b, CodecF32{}, v, false); break;
+					case PREC_FP16S: launch_even_v(b, CodecF16{s16, 1.0f / s16}, v, false); break;
+					default: launch_even_v(b, CodecF64{}, v, false);
+				}
+			} else {
+				switch (b->args.prec) {
+					case PREC_FP32S:
+						if (b->args.adjRle) launch_batch_odd<CodecF32, true>(b, CodecF32{}, false, false);
+						else launch_batch_odd<CodecF32, false>(b, CodecF32{}, false, false);
+						break;
+					case PREC_FP16S:
+						if (b->args.adjRle) launch_batch_odd<CodecF16, true>(b, CodecF16{s16, 1.0f / s16}, false, false);
+						else launch_batch_odd<CodecF16, false>(b, CodecF16{s16, 1.0f / s16}, false, false);
+						break;
+					default:
+						if (b->args.adjRle) launch_batch_odd<CodecF64, true>(b, CodecF64{}, false, false);
+						else launch_batch_odd<CodecF64, false>(b, CodecF64{}, false, false);
+				}
+			}
+		}
+		if (!hip_ok(hipGetLastError(), "tuning launch")) return false;
+		if (!hip_ok(hipEventRecord(t1, b->stream), "hipEventRecord")) return false;
+		if (!hip_ok(hipEventSynchronize(t1), "tuning")) return false;
+		return hip_ok(hipEventElapsedTime(ms, t0, t1), "hipEventElapsedTime");
+	};
+	for (const Cand & c : evens) {
+		b->tune = saved;
+		b->tune.evenBlock = c.block;
+		b->tune.evenCap = c.cap;
+		b->tune.vec = c.vec;
+		b->tablesDirty = true;
+		b->graphDirty = true;
+		float ms = 0.0f;
+		if (!(ok = time_it(true, &ms))) break;
+		if (ms < bestEven) { bestEven = ms; best.evenBlock = c.block; best.evenCap = c.cap; best.vec = c.vec; }
+	}
+	for (int o : odds) {
+		if (!ok) break;
+		b->tune = saved;
+		b->tune.oddBlock = o;
+		b->tablesDirty = true;
+		b->graphDirty = true;
+		float ms = 0.0f;
+		if (!(ok = time_it(false, &ms))) break;
+		if (ms < bestOdd) { bestOdd = ms; best.oddBlock = o; }
+	}
+	b->tune = ok ? best : saved;
+	b->tune.evenMs = ok ? bestEven / pairs : 0.0;
+	b->tune.oddMs = ok ? bestOdd / pairs : 0.0;
+	b->tablesDirty = true;
+	b->graphDirty = true;
+	// restore the lattices
+	for (BatchEntry & x : b->e)
+		if (x.state == 1) { x.L->c.shadowDirty = 1; x.L->c.shadowStale = 0; }
+	if (out) *out = b->tune;
+	return ok ? 0 : -1;
+}
+
+}  // extern "C"

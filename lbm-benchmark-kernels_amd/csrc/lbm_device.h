@@ -144,6 +144,7 @@ __device__ __forceinline__ void st(T * p, T v)
 struct CodecF64 {               // the reference's FP64 values
 	using S = double;
 	using R = double;
+	static constexpr int VMAX = 1;   // nodes per 8-byte access
 	__device__ __forceinline__ R dec(S v) const { return v; }
 	__device__ __forceinline__ S enc(R v) const { return v; }
 	template <bool NT> __device__ __forceinline__ R load(const S * p) const { return ld<NT>(p); }
@@ -153,6 +154,7 @@ struct CodecF64 {               // the reference's FP64 values
 struct CodecF32 {               // deviation f - w in FP32
 	using S = float;
 	using R = float;
+	static constexpr int VMAX = 2;
 	__device__ __forceinline__ R dec(S v) const { return v; }
 	__device__ __forceinline__ S enc(R v) const { return v; }
 	template <bool NT> __device__ __forceinline__ R load(const S * p) const { return ld<NT>(p); }
@@ -162,6 +164,7 @@ struct CodecF32 {               // deviation f - w in FP32
 struct CodecF16 {               // deviation f - w times a power of two, in FP16
 	using S = uint16_t;
 	using R = float;
+	static constexpr int VMAX = 4;
 	float scale, inv;            // 2^k and 2^-k: the scaling itself is exact
 	__device__ __forceinline__ R dec(S v) const { return __half2float(__ushort_as_half(v)) * inv; }
 	__device__ __forceinline__ S enc(R v) const { return (uint16_t)__half_as_ushort(__float2half_rn(v * scale)); }

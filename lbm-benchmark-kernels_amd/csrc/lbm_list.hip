@@ -22,8 +22,7 @@
 // 128-B line).  Adjacency transposed to [18][adjStride] uint32 so that lane
 // n of a wavefront reads entry d of node n next to lane n+1: every
 // adjacency load is one coalesced 256-B access per wavefront.
-#include "lbm_device.h"
-#include "lbm_runtime.h"
+#include "lbm_list.h"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -31,32 +30,6 @@
 #include <new>
 
 namespace lbm {
-
-struct ListData {
-	KernelData kd;              // first: callers cast KernelData* to the kernel's struct
-	Common c;
-	int dims[3] = {0, 0, 0};
-	int periodic[3] = {0, 0, 0};
-	int blk[3] = {0, 0, 0};
-	int nFluid = 0;
-	int nCells = 0;
-	int adjStride = 0;
-	int nRuns = 0;                // runs of consecutive adjacency (no forced breaks)
-	int nRunsWave = 0;            // runs with a forced break at every 64-node wavefront
-	int nWaves = 0;
-	int runStride = 0;
-	unsigned long long * dRunMask = nullptr;   // per wavefront: bit l = node 64w+l starts a run
-	uint32_t * dRunBase = nullptr;             // per wavefront: index of its first run
-	uint32_t * dRunAdj = nullptr;              // [18][runStride]: adjacency of each run's first node
-	uint32_t * dGrid = nullptr;
-	uint32_t * dCoords = nullptr;
-	uint32_t * dAdj = nullptr;
-	void * dPdf = nullptr;       // 19 * nCells values of the storage type
-	void * hPdf = nullptr;       // host shadow, same layout and type, == kd.Pdfs[0]
-	double * dMoments = nullptr; // reduction scratch: 2 per block + 2
-	int momentBlocks = 0;
-	uint32_t * hGrid = nullptr;  // cell -> node id, for GetNode/SetNode
-};
 
 static ListData * as_list(KernelData * kd) { return reinterpret_cast<ListData *>(kd); }
 static const ListData * as_list(const KernelData * kd) { return reinterpret_cast<const ListData *>(kd); }
@@ -274,142 +247,36 @@ __global__ __launch_bounds__(256) void k_run_compact(const uint32_t * __restrict
 	for (int d = 0; d < 18; ++d) runAdj[(size_t)d * runStride + r] = adj[(size_t)d * stride + n];
 }
 
-// Adjacency of node n for the odd step, from the full table or the runs.
-struct AdjFull {
-	const uint32_t * adj;
-	uint32_t stride;
-	__device__ __forceinline__ void fetch(uint32_t n, uint32_t (&a)[18]) const
-	{
-#pragma unroll
-		for (int d = 0; d < 18; ++d) a[d] = adj[(size_t)d * stride + n];
-	}
-};
-
-struct AdjRuns {
-	const unsigned long long * mask;
-	const uint32_t * base;
-	const uint32_t * runAdj;
-	uint32_t runStride;
-	__device__ __forceinline__ void fetch(uint32_t n, uint32_t (&a)[18]) const
-	{
-		const uint32_t w = n >> 6;
-		const int lane = (int)(n & 63u);
-		// run starts at or before this node; lane 0 of a wavefront always starts one
-		const unsigned long long upTo = mask[w] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-		const uint32_t r = base[w] + (uint32_t)__popcll(upTo) - 1u;
-		const uint32_t off = (uint32_t)(lane - (63 - __clzll(upTo)));
-#pragma unroll
-		for (int d = 0; d < 18; ++d) a[d] = runAdj[(size_t)d * runStride + r] + off;
-	}
-};
-
 // ---------------------------------------------------------------- steps
-// Even step: node n loads its 19 local values, collides and stores
-// direction d into its own slot inv(d).  FORCE_LOAD adds the body-force
-// increment of the preceding odd step to each loaded value.
-// NT streams the lattice with non-temporal loads and stores: every value is
-// touched once per step, so caching it only evicts useful lines.
-// Codec: storage type and compute precision (FP64 values, or deviations
-// f - w in FP32 / scaled FP16 computed in FP32).
+// The per-node bodies (even_node, even_nodes, odd_node) live in lbm_list.h,
+// shared with the multi-lattice batch kernels.
 template <class Codec, bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_list_even(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
                                                    TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
-	using R = typename Codec::R;
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
-	R f[19];
-#pragma unroll
-	for (int d = 0; d < 19; ++d) {
-		f[d] = cod.template load<NT>(&pdf[(size_t)d * nCells + n]);
-		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
-	}
-	trt_collide(f, k);
-#pragma unroll
-	for (int d = 0; d < 19; ++d) cod.template store<NT>(&pdf[(size_t)inv_dir(d) * nCells + n], f[d]);
+	even_node<Codec, FORCE_LOAD, NT>(pdf, cod, n, nCells, k, ft);
 }
 
-// Even step over V consecutive nodes per thread, for the narrow storage
-// types: one 8-byte access per direction carries V values (V = 2 for FP32,
-// 4 for FP16), so a wavefront moves as many bytes per instruction as in
-// FP64.  Needs nCells % V == 0 (every plane starts V-aligned); slots past
-// nFluid in the last group are read (padding, zero-filled at Init) but not
-// written.  Each node's arithmetic is that of k_list_even.
 template <class Codec, int V, bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_list_even_v(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
                                                      TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
-	using R = typename Codec::R;
-	using S = typename Codec::S;
-	using VT = VecS<S, V>;
 	const uint32_t n0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
 	if (n0 >= nFluid) return;
-	R f[V][19];
-#pragma unroll
-	for (int d = 0; d < 19; ++d) {
-		const VT x = ld<NT>(reinterpret_cast<const VT *>(&pdf[(size_t)d * nCells + n0]));
-#pragma unroll
-		for (int j = 0; j < V; ++j) {
-			f[j][d] = cod.dec(x[j]);
-			if (FORCE_LOAD) f[j][d] = f[j][d] + ft.f[d];
-		}
-	}
-#pragma unroll
-	for (int j = 0; j < V; ++j) trt_collide(f[j], k);
-	if (n0 + V <= nFluid) {
-#pragma unroll
-		for (int d = 0; d < 19; ++d) {
-			VT y;
-#pragma unroll
-			for (int j = 0; j < V; ++j) y[j] = cod.enc(f[j][d]);
-			st<NT>(reinterpret_cast<VT *>(&pdf[(size_t)inv_dir(d) * nCells + n0]), y);
-		}
-	} else {
-		const uint32_t m = nFluid - n0;
-#pragma unroll
-		for (int d = 0; d < 19; ++d)
-#pragma unroll
-			for (int j = 0; j < V; ++j)
-				if ((uint32_t)j < m) cod.template store<NT>(&pdf[(size_t)inv_dir(d) * nCells + n0 + j], f[j][d]);
-	}
+	even_nodes<Codec, V, FORCE_LOAD, NT>(pdf, cod, n0, nFluid, nCells, k, ft);
 }
 
-// Odd step: direction d arrives from slot adj[n][inv d] (the neighbour's
-// swapped store of the even step, or the node's own slot d for a
-// bounce-back link); the post-collision value goes to adj[n][d].  Every slot
-// is read and written by exactly one node, so the update is in place and
-// race free.  FORCE_STORE adds the force increment of this step to each
-// stored value: a value stored through a bounce-back entry is read back by
-// the same node as direction inv(d), any other by the neighbour as d.
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, class Adj>
 __global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec cod, Adj adjacency, uint32_t nFluid,
                                                   uint32_t nCells, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
-	using R = typename Codec::R;
 	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	uint32_t a[18];
 	adjacency.fetch(n, a);
-	R f[19];
-#pragma unroll
-	for (int d = 0; d < 18; ++d) {
-		f[d] = cod.template load<false>(&pdf[a[inv_dir(d)]]);
-		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
-	}
-	const size_t cSlot = (size_t)dC * nCells + n;
-	f[dC] = cod.template load<false>(&pdf[cSlot]);
-	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
-	trt_collide(f, k);
-#pragma unroll
-	for (int d = 0; d < 18; ++d) {
-		R v = f[d];
-		if (FORCE_STORE) {
-			const bool bounce = a[d] == (uint32_t)inv_dir(d) * nCells + n;
-			v = v + (bounce ? ft.f[inv_dir(d)] : ft.f[d]);
-		}
-		cod.template store<false>(&pdf[a[d]], v);
-	}
-	cod.template store<false>(&pdf[cSlot], FORCE_STORE ? f[dC] + ft.f[dC] : f[dC]);
+	odd_node<Codec, FORCE_LOAD, FORCE_STORE>(pdf, cod, a, n, nCells, k, ft);
 }
 
 // Node sums in the local layout (after a completed Kernel() call), in FP64:
@@ -484,12 +351,12 @@ static bool list_local_layout(const ListData * L)
 
 // Shadow element <-> node-view value.  Shifted storage holds f - w_d
 // (w_d = w_inv(d), so a slot keeps the same shift whichever node reads it).
-static const double kW[19] = {
+const double kW[19] = {
 	1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
 	1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
 	1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 3.0 };
 
-static size_t elem_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32S ? 4 : 2; }
+size_t elem_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32S ? 4 : 2; }
 
 static double shadow_get(const ListData * L, size_t i, int d)
 {
@@ -536,7 +403,7 @@ static void list_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	ListData * L = as_list(kd);
 	uint32_t n;
-	if (!list_node(L, x, y, z, &n)) return;
+	if (!list_node(L, x, y, z, &n) || !shadow_writable(&L->c)) return;
 	for (int d = 0; d < 19; ++d) shadow_set(L, (size_t)d * L->nCells + n, d, pdfs[d]);
 	L->c.shadowDirty = 1;
 }
@@ -553,13 +420,13 @@ static void list_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 {
 	ListData * L = as_list(kd);
 	uint32_t n;
-	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19) return;
+	if (!list_node(L, x, y, z, &n) || dir < 0 || dir >= 19 || !shadow_writable(&L->c)) return;
 	shadow_set(L, (size_t)dir * L->nCells + n, dir, pdf);
 	L->c.shadowDirty = 1;
 }
 
 // ------------------------------------------------------------- Kernel()
-static bool list_upload(ListData * L)
+bool list_upload(ListData * L)
 {
 	if (!L->c.shadowDirty) { L->c.t.uploadMs = 0.0; return true; }
 	const double t0 = (double)clock() / CLOCKS_PER_SEC;
@@ -571,12 +438,14 @@ static bool list_upload(ListData * L)
 	return true;
 }
 
-static bool list_download(ListData * L)
+bool list_download(ListData * L)
 {
 	if (L->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
 	if (!hip_ok(hipMemcpyAsync(L->hPdf, L->dPdf, elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells,
 	                           hipMemcpyDeviceToHost, L->c.stream), "download of the device lattice")) return false;
-	return hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize");
+	if (!hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize")) return false;
+	L->c.shadowStale = 0;
+	return true;
 }
 
 // Algorithmic bytes per fluid node of the odd step: 19 loads and 19 stores
@@ -593,7 +462,7 @@ static double list_even_bytes(const ListData * L) { return 2.0 * 19 * (double)el
 
 // Nodes per thread of the even step: 8-byte accesses for the narrow storage
 // types when the plane stride allows aligned vectors (-vec 1 turns it off).
-static int list_even_vec(const ListData * L)
+int list_even_vec(const ListData * L)
 {
 	const int v = L->c.args.prec == PREC_FP32S ? 2 : L->c.args.prec == PREC_FP16S ? 4 : 1;
 	if (!L->c.args.evenVec || L->nCells % v != 0) return 1;
@@ -723,6 +592,7 @@ static bool list_steps(KernelData * kd, const CaseData * cd)
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
+	c.shadowStale = L->nFluid > 0 && steps > 0;
 	if (c.shadowDownload) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
 		if (!list_download(L)) return false;
@@ -735,6 +605,11 @@ static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 {
 	(void)ld;
 	(void)list_steps(kd, cd);
+}
+
+ListData * list_data(KernelData * kd)
+{
+	return (kd && kd->Kernel == list_kernel) ? as_list(kd) : nullptr;
 }
 
 // Mass sum_n sum_d f and x momentum sum_n sum_d c_x,d f of the device
@@ -926,10 +801,12 @@ void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Pa
 	L->nCells = (int)nCells;
 	L->adjStride = ((L->nFluid + 255) / 256) * 256;
 
-	printf("# blocking               x: %3d y: %3d z: %3d\n", L->blk[0], L->blk[1], L->blk[2]);
-	printf("# lattice size:          %e MiB\n", nCells * 8.0 * 19 / 1024.0 / 1024.0);
-	printf("# fluid lattice size:    %e MiB\n", L->nFluid * 8.0 * 19 / 1024.0 / 1024.0);
-	printf("# index size:            %e MiB\n", L->nFluid * 4.0 * 18 / 1024.0 / 1024.0);
+	if (!L->c.args.quiet) {
+		printf("# blocking               x: %3d y: %3d z: %3d\n", L->blk[0], L->blk[1], L->blk[2]);
+		printf("# lattice size:          %e MiB\n", nCells * 8.0 * 19 / 1024.0 / 1024.0);
+		printf("# fluid lattice size:    %e MiB\n", L->nFluid * 8.0 * 19 / 1024.0 / 1024.0);
+		printf("# index size:            %e MiB\n", L->nFluid * 4.0 * 18 / 1024.0 / 1024.0);
+	}
 
 	if (!list_build(L, ld)) { list_release(L); return; }
 
@@ -947,9 +824,11 @@ void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Pa
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "init")) { list_release(L); return; }
 
 	const double even = list_even_bytes(L), odd = list_odd_bytes(L);
-	printf("# total fluid nodes: %d   consecutive blocks: %d   runs cut at wavefronts: %d\n",
-	       L->nFluid, L->nRuns, L->nRunsWave);
-	printf("# loop balance:          %.2f B/FLUP  even: %.2f B/FLUP  odd %.2f B/FLUP\n", (even + odd) / 2, even, odd);
+	if (!L->c.args.quiet) {
+		printf("# total fluid nodes: %d   consecutive blocks: %d   runs cut at wavefronts: %d\n",
+		       L->nFluid, L->nRuns, L->nRunsWave);
+		printf("# loop balance:          %.2f B/FLUP  even: %.2f B/FLUP  odd %.2f B/FLUP\n", (even + odd) / 2, even, odd);
+	}
 
 	KernelData * kd = &L->kd;
 	kd->Pdfs[0] = (PdfT *)L->hPdf;     // storage type given by -prec (double by default)

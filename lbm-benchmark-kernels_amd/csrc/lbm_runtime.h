@@ -46,6 +46,7 @@ struct KernelArgs {
 	int prec = 0;          // list: PDF storage, see enum Prec
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
 	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
+	int quiet = 0;         // no informational lines at Init
 };
 
 // PDF storage of the list kernel.
@@ -90,6 +91,7 @@ struct Common {
 	KernelArgs args;
 	int iteration = -1;         // parity of the last completed step
 	int shadowDirty = 0;        // SetNode touched the host shadow since the last upload
+	int shadowStale = 0;        // the device lattice is newer than the host shadow
 	int launchEvents = 1;
 	int shadowDownload = 1;
 	hipEvent_t * ev = nullptr;  // 2 per launch
@@ -98,6 +100,10 @@ struct Common {
 	LbmHipTimings t = {};
 	bool (*download)(KernelData * kd) = nullptr;   // device lattice -> host shadow
 };
+
+// SetNode may change the host shadow only while it holds the device's
+// state; otherwise it reports an error and returns false.
+bool shadow_writable(const Common * c);
 
 // Common state of a KernelData created by this library, or nullptr.
 Common * common_of(KernelData * kd);

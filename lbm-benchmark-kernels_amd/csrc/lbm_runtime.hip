@@ -188,6 +188,8 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "auto")) a->evenVec = 1;
 			else if (!strcmp(v, "1")) a->evenVec = 0;
 			else return fail("argument -vec requires auto or 1.");
+		} else if (is("-quiet") || is("--quiet")) {
+			a->quiet = 1;
 		} else if (is("-even-cap") || is("--even-cap")) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on or off.");
 			const char * v = p->KernelArgs[++i];
@@ -202,7 +204,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1]\n");
-			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off]\n");
+			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off] [-quiet]\n");
 			return fail("kernel help requested.");
 		} else {
 			return fail("unknown kernel parameter: %s.", s);
@@ -280,6 +282,13 @@ ForceTerms force_terms(double F)
 }
 
 // ----------------------------------------------------------- common state
+bool shadow_writable(const Common * c)
+{
+	if (!c->shadowStale) return true;
+	return fail("SetNode: the device lattice is newer than the host copy (shadow download off); "
+	            "call LbmHipSyncShadow first.");
+}
+
 bool common_init(Common * c, const Parameters * params, bool listKernel)
 {
 	if (!parse_kernel_args(params, &c->args, listKernel)) return false;

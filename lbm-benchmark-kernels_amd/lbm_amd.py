@@ -90,6 +90,17 @@ class LbmHipSteady(C.Structure):
                 ("mass", C.c_double), ("momentumX", C.c_double), ("seconds", C.c_double)]
 
 
+class LbmHipBatchStatus(C.Structure):
+    _fields_ = [("active", C.c_int), ("converged", C.c_int), ("steps", C.c_int), ("nFluid", C.c_int),
+                ("relChange", C.c_double), ("mass", C.c_double), ("momentumX", C.c_double),
+                ("seconds", C.c_double)]
+
+
+class LbmHipBatchTuning(C.Structure):
+    _fields_ = [("evenBlock", C.c_int), ("evenCap", C.c_int), ("vec", C.c_int), ("oddBlock", C.c_int),
+                ("evenMs", C.c_double), ("oddMs", C.c_double)]
+
+
 class LbmHipStreamResult(C.Structure):
     _fields_ = [("copyGBs", C.c_double), ("update19GBs", C.c_double), ("copyBytes", C.c_double),
                 ("update19Bytes", C.c_double)]
@@ -110,7 +121,10 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipGetTimings", "LbmHipSetLaunchEvents", "LbmHipSetShadowDownload",
                "LbmHipSyncShadow", "LbmHipListGetInfo", "LbmHipListCopyBuild", "LbmHipDumpNodes",
                "LbmHipLoadNodes", "LbmHipDeviceInfo", "LbmHipStreamBench", "LbmHipListMoments",
-               "LbmHipListRunSteady"]
+               "LbmHipListRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
+               "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
+               "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
+               "LbmHipBatchAutotune", "LbmHipBatchStats"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -170,6 +184,21 @@ def hip_lib() -> C.CDLL:
         L.LbmHipDeviceInfo.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.LbmHipStreamBench.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(LbmHipStreamResult)]
         L.LbmHipListMoments.argtypes = [KD, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.LbmHipBatchCreate.argtypes = [C.POINTER(Parameters)]
+        L.LbmHipBatchCreate.restype = C.c_void_p
+        L.LbmHipBatchDestroy.argtypes = [C.c_void_p]
+        L.LbmHipBatchAdd.argtypes = [C.c_void_p, C.POINTER(LatticeDesc), C.c_double, C.c_double, C.c_int]
+        L.LbmHipBatchRemove.argtypes = [C.c_void_p, C.c_int]
+        L.LbmHipBatchKernelData.argtypes = [C.c_void_p, C.c_int]
+        L.LbmHipBatchKernelData.restype = KD
+        L.LbmHipBatchStatusOf.argtypes = [C.c_void_p, C.c_int, C.POINTER(LbmHipBatchStatus)]
+        L.LbmHipBatchStep.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.LbmHipBatchRun.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.LbmHipBatchSync.argtypes = [C.c_void_p, C.c_int]
+        L.LbmHipBatchGetTuning.argtypes = [C.c_void_p, C.POINTER(LbmHipBatchTuning)]
+        L.LbmHipBatchSetTuning.argtypes = [C.c_void_p, C.POINTER(LbmHipBatchTuning)]
+        L.LbmHipBatchAutotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(LbmHipBatchTuning)]
+        L.LbmHipBatchStats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.LbmHipListRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
                                           C.POINTER(LbmHipSteady)]
         _libs["hip"] = L
@@ -389,6 +418,107 @@ class Kernel:
         if self.kd:
             self.kf.Deinit(C.byref(self.geo.ld), C.byref(self.kd))
             self.kd = C.POINTER(KernelData)()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many list-aa-pv-soa lattices stepped together (LbmHipBatch* of liblbmhip.so)."""
+
+    def __init__(self, kernel_args: Sequence[str] = (), errors: str = "return"):
+        L = hip_lib()
+        L.LbmHipSetErrorMode(LBM_HIP_ERRORS_RETURN if errors == "return" else LBM_HIP_ERRORS_EXIT)
+        L.LbmHipClearError()
+        words = [w.encode() for w in kernel_args]
+        self._argv = (C.c_char_p * max(1, len(words)))(*words)
+        self.params = Parameters(0, None, len(words), self._argv)
+        self.h = L.LbmHipBatchCreate(C.byref(self.params))
+        if not self.h:
+            raise RuntimeError(L.LbmHipLastError().decode() or "batch creation failed")
+        self.geos = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(hip_lib().LbmHipLastError().decode())
+
+    def add(self, geo: Geometry, omega: float, xforce: float, max_steps: int = 0) -> int:
+        L = hip_lib()
+        L.LbmHipClearError()
+        i = L.LbmHipBatchAdd(self.h, C.byref(geo.ld), omega, xforce, max_steps)
+        if i < 0:
+            raise RuntimeError(L.LbmHipLastError().decode())
+        self.geos[i] = geo
+        return i
+
+    def remove(self, i: int):
+        self._check(hip_lib().LbmHipBatchRemove(self.h, i))
+        self.geos.pop(i, None)
+
+    def step(self, steps: int, tol: float = 0.0) -> int:
+        n = C.c_int()
+        self._check(hip_lib().LbmHipBatchStep(self.h, steps, tol, C.byref(n)))
+        return n.value
+
+    def run(self, check_every: int, tol: float):
+        self._check(hip_lib().LbmHipBatchRun(self.h, check_every, tol))
+
+    def status(self, i: int) -> LbmHipBatchStatus:
+        st = LbmHipBatchStatus()
+        self._check(hip_lib().LbmHipBatchStatusOf(self.h, i, C.byref(st)))
+        return st
+
+    def kernel_data(self, i: int):
+        return hip_lib().LbmHipBatchKernelData(self.h, i)
+
+    def sync(self, i: int = -1):
+        self._check(hip_lib().LbmHipBatchSync(self.h, i))
+
+    def dump(self, i: int) -> np.ndarray:
+        """Node views of domain i (mask order), after copying it to the host."""
+        self.sync(i)
+        geo = self.geos[i]
+        out = np.zeros((geo.n_fluid, NDIR))
+        self._check(hip_lib().LbmHipDumpNodes(self.kernel_data(i), C.byref(geo.ld), out.ctypes.data))
+        return out
+
+    def moments(self, i: int):
+        m, j = C.c_double(), C.c_double()
+        self._check(hip_lib().LbmHipListMoments(self.kernel_data(i), C.byref(m), C.byref(j)))
+        return m.value, j.value
+
+    def tuning(self) -> LbmHipBatchTuning:
+        t = LbmHipBatchTuning()
+        self._check(hip_lib().LbmHipBatchGetTuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, even_block: int, even_cap: int, vec: int, odd_block: int):
+        t = LbmHipBatchTuning(even_block, even_cap, vec, odd_block, 0.0, 0.0)
+        self._check(hip_lib().LbmHipBatchSetTuning(self.h, C.byref(t)))
+
+    def autotune(self, pairs: int = 4) -> LbmHipBatchTuning:
+        t = LbmHipBatchTuning()
+        self._check(hip_lib().LbmHipBatchAutotune(self.h, pairs, C.byref(t)))
+        return t
+
+    def stats(self):
+        g, f = C.c_double(), C.c_double()
+        self._check(hip_lib().LbmHipBatchStats(self.h, C.byref(g), C.byref(f)))
+        return g.value, f.value
+
+    def close(self):
+        if self.h:
+            hip_lib().LbmHipBatchDestroy(self.h)
+            self.h = None
 
     def __enter__(self):
         return self

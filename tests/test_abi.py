@@ -142,3 +142,17 @@ def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):
         lbm.Kernel("aa-soa", geo, args)
+
+
+def test_batch_fails_loudly_without_device(lbm):
+    if not _no_gpu(lbm):
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        lbm.Batch(["-prec", "fp32s"])
+
+
+def test_batch_argument_errors(lbm):
+    with pytest.raises(RuntimeError, match="fp64, fp32s or fp16s"):
+        lbm.Batch(["-prec", "fp8"])
+    with pytest.raises(RuntimeError, match="unknown kernel parameter"):
+        lbm.Batch(["-foo"])

@@ -1,0 +1,131 @@
+"""GPU tests of the multi-lattice batch (LbmHipBatch*): many list-aa-pv-soa
+domains of different sizes, geometries, omegas and forces stepped together by
+one persistent launch per half-step, replayed from a HIP graph.
+
+Bar: every domain's node views after the batch equal, bit for bit, the same
+lattice run alone through the plugin's Kernel() for the same number of steps
+(and the oracle's result), whatever the launch shape; moments and the
+convergence bookkeeping agree with the node views.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM = 3 / 16
+DOMAINS = [("porous-2-0.7-5", (20, 18, 16), 1.2, 1e-5),
+           ("channel", (14, 12, 10), 0.9, 2e-5),
+           ("porous-3-0.75-9", (40, 36, 32), 1.5, 1e-5),
+           ("box", (12, 12, 12), 1.0, 3e-5),
+           ("blocks-2", (18, 16, 14), 1.3, 1e-5)]
+
+
+def _geos(lbm):
+    return [(lbm.Geometry.from_string(g, dims), om, F) for g, dims, om, F in DOMAINS]
+
+
+def _alone(lbm, geo, om, F, runs, args):
+    with lbm.Kernel("list-aa-pv-soa", geo, list(args)) as k:
+        k.set_initial_weights()
+        for s in runs:
+            k.run(lbm.case_data(omega=om, xforce=F, iterations=s))
+        return k.dump()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s", "fp16s"])
+def test_batch_equals_single_lattice(lbm, oracle, prec):
+    args = ["-prec", prec, "-lambda", repr(LAM)]
+    geos = _geos(lbm)
+    with lbm.Batch(args) as b:
+        ids = [b.add(g, om, F) for g, om, F in geos]
+        for _ in range(3):
+            assert b.step(20) == len(ids)
+        for i, (g, om, F) in zip(ids, geos):
+            print(f"batch {prec} domain {i} dims={g.dims}", flush=True)
+            got = b.dump(i)
+            st = b.status(i)
+            assert st.steps == 60 and st.active == 1
+            # fp16s rounds the force increment into storage at every interval end
+            runs = [20, 20, 20] if prec == "fp16s" else [60]
+            assert np.array_equal(got, _alone(lbm, g, om, F, runs, args)), i
+            m = g.mask.reshape(-1)
+            if prec == "fp64":
+                want, _ = oracle.list_run(m, g.dims, g.periodic, 60, om, F, magic=LAM, vsize=1)
+            else:
+                want, _ = oracle.list_run_shifted(m, g.dims, g.periodic, runs, om, F, prec, magic=LAM)
+            assert np.array_equal(got, want), i
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s", "fp16s"])
+def test_batch_launch_shapes(lbm, prec):
+    args = ["-prec", prec, "-lambda", repr(LAM)]
+    geos = _geos(lbm)[:3]
+    vmax = {"fp64": 1, "fp32s": 2, "fp16s": 4}[prec]
+    want = [_alone(lbm, g, om, F, [40] if prec != "fp16s" else [20, 20], args) for g, om, F in geos]
+    shapes = [(512, 1, vmax, 256), (256, 0, 1, 128), (512, 0, vmax, 512), (64, 0, 1, 64), (256, 1, vmax, 256)]
+    for shape in shapes + ["auto"]:
+        print(f"batch {prec} shape {shape}", flush=True)
+        with lbm.Batch(args + ["-adj", "full"] if shape == (64, 0, 1, 64) else args) as b:
+            ids = [b.add(g, om, F) for g, om, F in geos]
+            if shape == "auto":
+                t = b.autotune(2)
+                assert t.evenMs > 0 and t.oddMs > 0 and 1 <= t.vec <= vmax
+            else:
+                b.set_tuning(*shape)
+            b.step(20)
+            b.step(20)
+            for i, w in zip(ids, want):
+                assert np.array_equal(b.dump(i), w), (shape, i)
+
+
+def test_batch_membership_and_convergence(lbm):
+    args = ["-lambda", repr(LAM)]
+    geos = _geos(lbm)
+    with lbm.Batch(args) as b:
+        a = b.add(*geos[0], max_steps=40)
+        c = b.add(*geos[1], max_steps=0)
+        d = b.add(*geos[2], max_steps=100)
+        assert b.step(20) == 3
+        assert b.step(20) == 2                     # a reached its 40 steps
+        assert (b.status(a).active, b.status(a).steps) == (0, 40)
+        b.remove(c)                                # leaves mid-run
+        e = b.add(*geos[3], max_steps=40)          # joins mid-run, reuses the free slot
+        assert e == c
+        b.step(20)
+        b.step(20)
+        st_d, st_e = b.status(d), b.status(e)
+        assert (st_d.steps, st_e.steps, st_e.active) == (80, 40, 0)
+        for i, (g, om, F), steps in ((a, geos[0], 40), (d, geos[2], 80), (e, geos[3], 40)):
+            assert np.array_equal(b.dump(i), _alone(lbm, g, om, F, [steps], args)), i
+        # moments of the status equal those of the node views
+        dd = b.dump(d)
+        m, j = b.moments(d)
+        assert st_d.mass == pytest.approx(dd.sum(), rel=1e-12) and st_d.mass == pytest.approx(m, rel=1e-13)
+        assert st_d.momentumX == pytest.approx(j, rel=1e-12)
+        gpu_s, flups = b.stats()
+        assert gpu_s > 0 and flups == sum(geo.n_fluid * s for (geo, _, _), s in
+                                          ((geos[0], 40), (geos[1], 40), (geos[2], 80), (geos[3], 40)))
+    # a loose tolerance: every domain converges before its limit
+    with lbm.Batch(["-prec", "fp32s"]) as b:
+        ids = [b.add(g, om, F, max_steps=40000) for g, om, F in geos[:2]]
+        b.run(200, 1e-4)
+        for i in ids:
+            st = b.status(i)
+            assert st.converged == 1 and st.active == 0 and 0 <= st.relChange <= 1e-4 and st.steps < 40000
+
+
+def test_batch_setnode_guard(lbm):
+    g, om, F = _geos(lbm)[0]
+    with lbm.Batch() as b:
+        i = b.add(g, om, F)
+        b.step(10)
+        kd = b.kernel_data(i)
+        vals = (lbm.PdfT * 19)(*([0.05] * 19))
+        lbm.hip_lib().LbmHipClearError()
+        x, y, z = [int(v) for v in np.argwhere(g.mask != 0)[0][::-1]]
+        kd.contents.SetNode(kd, x, y, z, vals)     # host copy is stale: refused
+        assert "newer than the host copy" in lbm.hip_lib().LbmHipLastError().decode()
+        b.sync(i)
+        lbm.hip_lib().LbmHipClearError()
+        kd.contents.SetNode(kd, x, y, z, vals)
+        assert lbm.hip_lib().LbmHipLastError().decode() == ""
