@@ -77,6 +77,184 @@ def cpu_baseline(mask, dims, periodic, n_fluid, steps):
             "sample": f"C restatement, {steps} steps incl. list build, OpenMP {threads} threads"}
 
 
+def _log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _common(metric, value, unit, steps, workload, extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic", "config": {"workload": workload}}
+    line.update(extra)
+    return line
+
+
+def batch_mode(args, lbm):
+    """Many seeded porous samples of uneven size solved to steady state (or the
+    step limit) in one batch, FP64 and shifted FP32; a stride subset solved
+    one at a time through the single-lattice FP64 path for comparison."""
+    import workloads as W
+
+    samples = W.batch_samples(args.samples, args.seed)
+    t0 = time.perf_counter()
+    geos = [lbm.Geometry.from_string(s.geometry, s.dims) for s in samples]
+    _log(f"{len(samples)} samples generated in {time.perf_counter() - t0:.1f} s, "
+         f"{sum(g.n_fluid for g in geos) / 1e6:.1f} M fluid nodes")
+    kargs = ["-lambda", repr(LAMBDA), "-pad", "16384+128"]
+    res = {}
+    for prec in ("fp64", "fp32s"):
+        with lbm.Batch(kargs + ["-prec", prec]) as b:
+            t0 = time.perf_counter()
+            ids = [b.add(g, s.omega, s.xforce, args.max_steps) for g, s in zip(geos, samples)]
+            setup = time.perf_counter() - t0
+            tune = b.autotune(4)
+            _log(f"{prec}: {len(ids)} domains set up in {setup:.1f} s; tuning even {tune.evenBlock}/cap "
+                 f"{tune.evenCap}/vec {tune.vec} odd {tune.oddBlock}")
+            t0 = time.perf_counter()
+            active, rounds = len(ids), 0
+            while active:
+                active = b.step(args.check, args.tol)
+                rounds += 1
+                if rounds % 10 == 0:
+                    _log(f"{prec}: {rounds * args.check} steps, {active} active")
+            wall = time.perf_counter() - t0
+            gpu_s, flups = b.stats()
+            st = [b.status(i) for i in ids]
+            perm = [W.permeability(x.momentumX, x.nFluid, s.cells, s.omega, s.xforce) for x, s in zip(st, samples)]
+            res[prec] = {"solves_per_s": round(len(ids) / wall, 3), "wall_s": round(wall, 3),
+                         "gpu_s": round(gpu_s, 3), "setup_s": round(setup, 2),
+                         "mflups": round(flups / wall / 1e6, 1),
+                         "converged": sum(x.converged for x in st),
+                         "steps_median": W.percentile([x.steps for x in st], 50),
+                         "steps_max": max(x.steps for x in st),
+                         "tuning": {"even_block": tune.evenBlock, "even_cap": tune.evenCap, "vec": tune.vec,
+                                    "odd_block": tune.oddBlock, "even_ms": round(tune.evenMs, 4),
+                                    "odd_ms": round(tune.oddMs, 4)},
+                         "_perm": perm, "_steps": [x.steps for x in st]}
+            _log(f"{prec}: {res[prec]['solves_per_s']} solves/s, {res[prec]['mflups']} MFLUP/s, "
+                 f"{res[prec]['converged']} converged")
+    err = [abs(a - b) / abs(b) for a, b in zip(res["fp32s"]["_perm"], res["fp64"]["_perm"])]
+    # the plain path: one lattice at a time, FP64, same stopping rule
+    sub = list(range(0, len(samples), max(1, args.baseline_stride)))
+    t_run = 0.0
+    for i in sub:
+        s, g = samples[i], geos[i]
+        with lbm.Kernel("list-aa-pv-soa", g, kargs + ["-quiet"]) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            cd = lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0)
+            t0 = time.perf_counter()
+            k.run_steady(cd, args.check, args.tol, args.max_steps)
+            t_run += time.perf_counter() - t0
+    base = len(sub) / t_run
+    _log(f"per-request fp64: {len(sub)} samples in {t_run:.1f} s -> {base:.2f} solves/s")
+    for r in res.values():
+        r.pop("_perm")
+        r.pop("_steps")
+    v = res["fp32s"]["solves_per_s"]
+    return _common(
+        f"solves/s ({len(samples)} porous samples, batched list-AA, shifted FP32)", v, "solves/s", args.max_steps,
+        f"{len(samples)} porous samples, edges log-uniform in [32,128], porosity U[0.5,0.9], sphere radius U[3,8], "
+        f"omega U[0.8,1.6], F 1e-6, Lambda 3/16, seed {args.seed}; to relative x-momentum change < {args.tol:g} "
+        f"per {args.check} steps or {args.max_steps} steps",
+        {"batch": res,
+         "per_request_fp64": {"solves_per_s": round(base, 3), "samples_timed": len(sub),
+                              "note": "every %d-th sample solved alone through Kernel() (run time only)"
+                                      % args.baseline_stride},
+         "speedup_fp32s_batch_vs_per_request_fp64": round(v / base, 2),
+         "speedup_fp64_batch_vs_per_request_fp64": round(res["fp64"]["solves_per_s"] / base, 2),
+         "permeability_rel_err_fp32s_vs_fp64": {"max": max(err), "p99": W.percentile(err, 99),
+                                                "median": W.percentile(err, 50)}})
+
+
+def precision_mode(args, lbm):
+    """The 64^3 channel (x periodic, walls on y and z) to steady state in
+    FP64, shifted FP32 and shifted FP16: velocity-field and permeability
+    error against FP64."""
+    import numpy as np
+    import workloads as W
+
+    dims, omega, F, tol = (64, 64, 64), 1.0, 1e-5, 1e-7
+    geo = lbm.Geometry.from_string("channel", dims, (1, 0, 0))   # -periodic-x, as the reference's driver takes it
+    cx = np.array([0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0], dtype=float)
+    out, ref = {}, None
+    for prec in ("fp64", "fp32s", "fp16s"):
+        with lbm.Kernel("list-aa-pv-soa", geo, ["-lambda", repr(LAMBDA), "-prec", prec, "-quiet"]) as k:
+            k.set_initial_weights()
+            r = k.run_steady(lbm.case_data(omega=omega, xforce=F, iterations=0), args.check, tol, 200000)
+            d = k.dump()
+        u = d @ cx - F / 2
+        perm = W.permeability(r.momentumX, geo.n_fluid, int(np.prod(dims)), omega, F)
+        out[prec] = {"steps": r.steps, "converged": r.converged, "rel_change": r.relChange,
+                     "mflups": round(geo.n_fluid * r.steps / r.seconds / 1e6, 1), "permeability": perm,
+                     "mass_rel_drift": abs(r.mass - geo.n_fluid) / geo.n_fluid}
+        if ref is None:
+            ref = (u, perm)
+        else:
+            out[prec]["velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0]))
+            out[prec]["permeability_rel_err_vs_fp64"] = abs(perm - ref[1]) / abs(ref[1])
+        _log(f"{prec}: {out[prec]}")
+    return _common("velocity rel. L2 vs FP64 at steady state (64^3 channel, shifted FP16)",
+                   out["fp16s"]["velocity_rel_l2_vs_fp64"], "rel", out["fp64"]["steps"],
+                   "list-aa-pv-soa, 64^3 channel (x periodic, walls y/z), omega 1.0, Lambda 3/16, F 1e-5, "
+                   f"to relative x-momentum change < {tol:g} per {args.check} steps",
+                   {"higher_is_better": False, "precision": out})
+
+
+def serve_mode(args, lbm):
+    """Poisson streams of permeability requests (porous cubes, edge U[32,192],
+    FP64 or shifted-FP32 tag) through the continuous-batching server, one
+    stream per rate; the plain path solves the same requests one at a time in
+    FP64 (service times measured once, queueing by Lindley's recursion)."""
+    import serving as S
+    import workloads as W
+
+    rates = [float(r) for r in args.rates.split(",")]
+    samples = S.serving_samples(args.requests, args.seed)
+    kargs = ["-lambda", repr(LAMBDA), "-pad", "16384+128"]
+    max_steps, tol, interval = 20000, args.tol, args.check
+    service, perm64, flups_plain = [], [], 0.0
+    for s, _ in samples:
+        t0 = time.perf_counter()
+        geo = lbm.Geometry.from_string(s.geometry, s.dims)
+        with lbm.Kernel("list-aa-pv-soa", geo, kargs + ["-quiet"]) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            r = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), interval, tol, max_steps)
+        service.append(time.perf_counter() - t0)
+        perm64.append(W.permeability(r.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce))
+        flups_plain += geo.n_fluid * r.steps
+    _log(f"plain fp64 path: {len(samples)} requests, {sum(service):.1f} s of service")
+    out = {}
+    for rate in rates:
+        arr = S.poisson_arrivals(len(samples), rate, args.seed + int(rate))
+        reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arr, samples))]
+        be = S.GpuBackend(lbm, kargs, max_steps)
+        try:
+            recs = S.serve(reqs, be, int(args.budget_gb * 1e9), interval, tol)
+            summ = S.summarize(recs, be.flups())
+        finally:
+            be.close()
+        fin = S.fifo_latencies(arr, service)
+        lat = [f - a for f, a in zip(fin, arr)]
+        span = fin[-1] - arr[0]
+        plain = {"p50_latency_s": round(W.percentile(lat, 50), 4), "p99_latency_s": round(W.percentile(lat, 99), 4),
+                 "solves_per_s": round(len(arr) / span, 3), "mflups": round(flups_plain / span / 1e6, 1)}
+        err = [abs(r.permeability - p) / abs(p) for r, p in zip(recs, perm64) if r.prec == "fp32s" and not r.rejected]
+        summ["permeability_rel_err_fp32s_vs_fp64_max"] = max(err) if err else None
+        summ["fp32s_requests"] = len(err)
+        out[str(int(rate))] = {"served": summ, "plain_fp64": plain,
+                               "solves_speedup": round(summ["solves_per_s"] / plain["solves_per_s"], 2)}
+        _log(f"rate {rate:g}/s: served {summ} | plain {plain}")
+    top = out[str(int(max(rates)))]
+    return _common(f"solves/s (served permeability requests, Poisson {max(rates):g}/s)",
+                   top["served"]["solves_per_s"], "solves/s", max_steps,
+                   f"{len(samples)} requests per rate: porous cubes, edge U[32,192], porosity U[0.5,0.9], sphere "
+                   f"radius U[3,8], omega U[0.8,1.6], F 1e-6, FP64 or shifted-FP32 tag (50/50), to relative x-momentum "
+                   f"change < {tol:g} per {interval} steps (at most {max_steps}); device budget {args.budget_gb:g} GB",
+                   {"p99_latency_s": top["served"]["p99_latency_s"], "serving": out})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +264,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prec", choices=["fp64", "fp32s", "fp16s"], default="fp64",
                     help="PDF storage: fp64 (the headline), or deviations f - w in FP32 / scaled FP16")
+    ap.add_argument("--mode", choices=["list", "batch", "precision", "serve"], default="list",
+                    help="list: the 256^3 headline; batch: many porous samples in one batch; "
+                         "precision: the 64^3 channel to steady state in every storage type; "
+                         "serve: Poisson request streams through the continuous-batching server")
+    ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=32, help="serve: requests per rate")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget (GB)")
+    ap.add_argument("--samples", type=int, default=512, help="batch: number of porous samples")
+    ap.add_argument("--seed", type=int, default=2024, help="batch: sample generator seed")
+    ap.add_argument("--max-steps", type=int, default=5000, help="batch: step limit per sample")
+    ap.add_argument("--check", type=int, default=100, help="batch/precision: steps between convergence checks")
+    ap.add_argument("--tol", type=float, default=1e-6, help="batch: relative change of the x momentum per check")
+    ap.add_argument("--baseline-stride", type=int, default=8,
+                    help="batch: every n-th sample is also solved alone (the per-request FP64 path)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (its share of the lattice at the real shape); "
@@ -98,6 +290,12 @@ def main():
     os.dup2(2, 1)
 
     import lbm_amd as lbm
+
+    if args.mode != "list":
+        line = {"batch": batch_mode, "precision": precision_mode, "serve": serve_mode}[args.mode](args, lbm)
+        os.dup2(json_fd, 1)
+        print(json.dumps(line), flush=True)
+        return
 
     steps = args.steps + (args.steps & 1)
     warm = args.warmup + (args.warmup & 1)

@@ -194,6 +194,12 @@ int LbmHipBatchStatusOf(LbmHipBatch * b, int id, LbmHipBatchStatus * out);
 /* One interval of `steps` (rounded up to even) for all active domains;
  * *nActive receives the number still active afterwards. */
 int LbmHipBatchStep(LbmHipBatch * b, int steps, double tol, int * nActive);
+/* The two halves of LbmHipBatchStep: Launch starts the interval on the
+ * batch's stream and returns; Wait completes it (convergence test, status).
+ * Between them the batch accepts no other call; other batches and host work
+ * may proceed, so intervals of several batches overlap on the device. */
+int LbmHipBatchLaunch(LbmHipBatch * b, int steps);
+int LbmHipBatchWait(LbmHipBatch * b, double tol, int * nActive);
 /* Intervals of checkEvery steps until no domain is active. */
 int LbmHipBatchRun(LbmHipBatch * b, int checkEvery, double tol);
 /* Device -> host copy of one domain (id >= 0) or of all (id < 0). */

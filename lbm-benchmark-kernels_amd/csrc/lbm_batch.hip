@@ -202,6 +202,9 @@ struct Batch {
 	bool graphDirty = true;
 
 	double gpuMs = 0.0, flups = 0.0;
+	int pendingSteps = 0;          // an interval launched and not yet waited for
+	double * hSumsPinned = nullptr;
+	size_t hSumsCap = 0;
 };
 
 static Batch * as_batch(LbmHipBatch * b) { return reinterpret_cast<Batch *>(b); }
@@ -446,13 +449,20 @@ static void launch_moments(Batch * b, Codec cod)
 	k_batch_moments<Codec><<<(unsigned)b->hMom.size(), 256, 0, b->stream>>>((const BatchDom<R> *)b->dDoms, b->dMom, cod, b->dPartial);
 }
 
-// FP64 mass and x momentum of every active domain into its status.
-static bool batch_moments(Batch * b)
+// FP64 mass and x momentum of every active domain: kernels and an
+// asynchronous copy into pinned host memory, read by batch_finish.
+static bool batch_moments_launch(Batch * b)
 {
 	const size_t nA = b->activeIds.size();
 	if (nA == 0) return true;
+	if (b->hSumsCap < 2 * nA) {
+		host_free(b->hSumsPinned);
+		b->hSumsCap = 0;
+		b->hSumsPinned = (double *)host_alloc(sizeof(double) * 2 * nA);
+		if (!b->hSumsPinned) return fail("allocating %zu bytes of pinned memory failed.", sizeof(double) * 2 * nA);
+		b->hSumsCap = 2 * nA;
+	}
 	const float s16 = f16_scale(b);
-	(void)hipGetLastError();
 	if (!b->hMom.empty()) {
 		switch (b->args.prec) {
 			case PREC_FP32S: launch_moments(b, CodecF32{}); break;
@@ -461,20 +471,15 @@ static bool batch_moments(Batch * b)
 		}
 	}
 	k_batch_reduce<<<(unsigned)nA, 256, 0, b->stream>>>(b->dSeg, b->dPartial, b->dSums);
-	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
-	b->hSums.resize(2 * nA);
-	if (!hip_ok(hipMemcpyAsync(b->hSums.data(), b->dSums, sizeof(double) * 2 * nA, hipMemcpyDeviceToHost, b->stream), "moment download")) return false;
-	if (!hip_ok(hipStreamSynchronize(b->stream), "moment kernels")) return false;
-	for (size_t i = 0; i < nA; ++i) {
-		BatchEntry & x = b->e[b->activeIds[i]];
-		x.st.mass = b->hSums[2 * i] + (b->args.prec == PREC_FP64 ? 0.0 : (double)x.L->nFluid);
-		x.st.momentumX = b->hSums[2 * i + 1];
-	}
-	return true;
+	return hip_ok(hipMemcpyAsync(b->hSumsPinned, b->dSums, sizeof(double) * 2 * nA, hipMemcpyDeviceToHost, b->stream),
+	              "moment download");
 }
 
-static bool batch_step(Batch * b, int steps, double tol)
+// Starts one interval of `steps` for all active domains (graph replay, then
+// the moment kernels); returns without waiting.
+static bool batch_launch(Batch * b, int steps)
 {
+	if (b->pendingSteps) return fail("LbmHipBatchLaunch: the previous interval has not been waited for.");
 	steps += steps & 1;
 	bool any = false;
 	for (BatchEntry & x : b->e) {
@@ -491,12 +496,26 @@ static bool batch_step(Batch * b, int steps, double tol)
 	if (!hip_ok(hipEventRecord(b->ev[0], b->stream), "hipEventRecord")) return false;
 	if (!hip_ok(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch")) return false;
 	if (!hip_ok(hipEventRecord(b->ev[1], b->stream), "hipEventRecord")) return false;
-	if (!batch_moments(b)) return false;
+	if (!batch_moments_launch(b)) return false;
+	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
+	b->pendingSteps = steps;
+	return true;
+}
+
+// Waits for the launched interval and applies the convergence test.
+static bool batch_finish(Batch * b, double tol)
+{
+	const int steps = b->pendingSteps;
+	if (!steps) return true;
+	b->pendingSteps = 0;
+	if (!hip_ok(hipStreamSynchronize(b->stream), "batch interval")) return false;
 	float ms = 0.0f;
 	if (!hip_ok(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "hipEventElapsedTime")) return false;
 	b->gpuMs += ms;
-	for (int id : b->activeIds) {
-		BatchEntry & x = b->e[id];
+	for (size_t i = 0; i < b->activeIds.size(); ++i) {
+		BatchEntry & x = b->e[b->activeIds[i]];
+		x.st.mass = b->hSumsPinned[2 * i] + (b->args.prec == PREC_FP64 ? 0.0 : (double)x.L->nFluid);
+		x.st.momentumX = b->hSumsPinned[2 * i + 1];
 		b->flups += (double)x.L->nFluid * steps;
 		x.st.steps += steps;
 		x.st.seconds += ms / 1e3;
@@ -519,6 +538,11 @@ static bool batch_step(Batch * b, int steps, double tol)
 	return true;
 }
 
+static bool batch_step(Batch * b, int steps, double tol)
+{
+	return batch_launch(b, steps) && batch_finish(b, tol);
+}
+
 static void batch_free_tables(Batch * b)
 {
 	drop_graph(b);
@@ -530,6 +554,13 @@ static void batch_free_tables(Batch * b)
 	(void)hipFree(b->dSeg);
 	(void)hipFree(b->dPartial);
 	(void)hipFree(b->dSums);
+}
+
+static bool idle(const Batch * b, const char * who)
+{
+	if (!b) return fail("%s: invalid batch.", who);
+	if (b->pendingSteps) return fail("%s: an interval is running; call LbmHipBatchWait first.", who);
+	return true;
 }
 
 static BatchEntry * entry(Batch * b, int id, const char * who)
@@ -591,7 +622,9 @@ void LbmHipBatchDestroy(LbmHipBatch * handle)
 	if (!b) return;
 	for (BatchEntry & x : b->e)
 		if (x.kd) D3Q19ListAaPvDeinit_PushSoAHip(nullptr, &x.kd);
+	if (b->pendingSteps) (void)hipStreamSynchronize(b->stream);
 	batch_free_tables(b);
+	host_free(b->hSumsPinned);
 	if (b->ev[0]) (void)hipEventDestroy(b->ev[0]);
 	if (b->ev[1]) (void)hipEventDestroy(b->ev[1]);
 	if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -602,6 +635,7 @@ int LbmHipBatchAdd(LbmHipBatch * handle, LatticeDesc * ld, double omega, double 
 {
 	Batch * b = as_batch(handle);
 	if (!b || !ld) { fail("LbmHipBatchAdd: invalid arguments."); return -1; }
+	if (!idle(b, "LbmHipBatchAdd")) return -1;
 	if (!(omega > 0.0 && omega < 2.0)) { fail("omega must be in (0, 2), got %f.", omega); return -1; }
 	KernelData * kd = nullptr;
 	D3Q19ListAaPvInit_PushSoAHip(ld, &kd, &b->params);
@@ -636,6 +670,7 @@ int LbmHipBatchAdd(LbmHipBatch * handle, LatticeDesc * ld, double omega, double 
 int LbmHipBatchRemove(LbmHipBatch * handle, int id)
 {
 	Batch * b = as_batch(handle);
+	if (!idle(b, "LbmHipBatchRemove")) return -1;
 	BatchEntry * x = entry(b, id, "LbmHipBatchRemove");
 	if (!x) return -1;
 	D3Q19ListAaPvDeinit_PushSoAHip(nullptr, &x->kd);
@@ -671,6 +706,26 @@ int LbmHipBatchStep(LbmHipBatch * handle, int steps, double tol, int * nActive)
 	return 0;
 }
 
+int LbmHipBatchLaunch(LbmHipBatch * handle, int steps)
+{
+	Batch * b = as_batch(handle);
+	if (!b || steps < 0) { fail("LbmHipBatchLaunch: invalid arguments."); return -1; }
+	return batch_launch(b, steps) ? 0 : -1;
+}
+
+int LbmHipBatchWait(LbmHipBatch * handle, double tol, int * nActive)
+{
+	Batch * b = as_batch(handle);
+	if (!b || !(tol >= 0.0)) { fail("LbmHipBatchWait: invalid arguments."); return -1; }
+	if (!batch_finish(b, tol)) return -1;
+	if (nActive) {
+		int k = 0;
+		for (const BatchEntry & x : b->e) k += x.state == 1;
+		*nActive = k;
+	}
+	return 0;
+}
+
 int LbmHipBatchRun(LbmHipBatch * handle, int checkEvery, double tol)
 {
 	Batch * b = as_batch(handle);
@@ -689,6 +744,7 @@ int LbmHipBatchRun(LbmHipBatch * handle, int checkEvery, double tol)
 int LbmHipBatchSync(LbmHipBatch * handle, int id)
 {
 	Batch * b = as_batch(handle);
+	if (!idle(b, "LbmHipBatchSync")) return -1;
 	if (id >= 0) {
 		BatchEntry * x = entry(b, id, "LbmHipBatchSync");
 		return (x && (!x->L->c.shadowStale || list_download(x->L))) ? 0 : -1;
@@ -712,6 +768,7 @@ int LbmHipBatchSetTuning(LbmHipBatch * handle, const LbmHipBatchTuning * t)
 {
 	Batch * b = as_batch(handle);
 	if (!b || !t) { fail("LbmHipBatchSetTuning: invalid arguments."); return -1; }
+	if (!idle(b, "LbmHipBatchSetTuning")) return -1;
 	const bool okBlock = t->evenBlock >= 64 && t->evenBlock <= 512 && t->evenBlock % 64 == 0 &&
 	                     t->oddBlock >= 64 && t->oddBlock <= 512 && t->oddBlock % 64 == 0;
 	if (!okBlock || (t->vec != 1 && t->vec != 2 && t->vec != 4) || t->vec > vec_max(b->args.prec)) {
@@ -742,6 +799,7 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 {
 	Batch * b = as_batch(handle);
 	if (!b || pairs <= 0) { fail("LbmHipBatchAutotune: invalid arguments."); return -1; }
+	if (!idle(b, "LbmHipBatchAutotune")) return -1;
 	for (BatchEntry & x : b->e) {
 		if (x.state != 1) continue;
 		if (x.L->c.shadowDirty) { if (!list_upload(x.L)) return -1; }

@@ -124,7 +124,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipListRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
-               "LbmHipBatchAutotune", "LbmHipBatchStats"]
+               "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -194,6 +194,8 @@ def hip_lib() -> C.CDLL:
         L.LbmHipBatchStatusOf.argtypes = [C.c_void_p, C.c_int, C.POINTER(LbmHipBatchStatus)]
         L.LbmHipBatchStep.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.LbmHipBatchRun.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.LbmHipBatchLaunch.argtypes = [C.c_void_p, C.c_int]
+        L.LbmHipBatchWait.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int)]
         L.LbmHipBatchSync.argtypes = [C.c_void_p, C.c_int]
         L.LbmHipBatchGetTuning.argtypes = [C.c_void_p, C.POINTER(LbmHipBatchTuning)]
         L.LbmHipBatchSetTuning.argtypes = [C.c_void_p, C.POINTER(LbmHipBatchTuning)]
@@ -467,6 +469,14 @@ class Batch:
     def step(self, steps: int, tol: float = 0.0) -> int:
         n = C.c_int()
         self._check(hip_lib().LbmHipBatchStep(self.h, steps, tol, C.byref(n)))
+        return n.value
+
+    def launch(self, steps: int):
+        self._check(hip_lib().LbmHipBatchLaunch(self.h, steps))
+
+    def wait(self, tol: float = 0.0) -> int:
+        n = C.c_int()
+        self._check(hip_lib().LbmHipBatchWait(self.h, tol, C.byref(n)))
         return n.value
 
     def run(self, check_every: int, tol: float):

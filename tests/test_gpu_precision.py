@@ -138,7 +138,7 @@ def test_run_steady(lbm, oracle):
 
 
 def _ux_perm(d, n_cells, omega, F):
-    ux = d @ CX + F / 2                      # Luo forcing: physical velocity
+    ux = d @ CX - F / 2                      # node views hold the last step's force increment
     nu = (1 / omega - 0.5) / 3
     return ux, nu * ux.sum() / n_cells / F   # Darcy: k = nu * U_superficial / F
 

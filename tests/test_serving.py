@@ -1,0 +1,147 @@
+"""CPU tests of the serving loop's admission and scheduling (a simulated
+backend stands in for the GPU), of the workload generators and of the
+permeability formula (against plane Poiseuille flow run on the oracle)."""
+import math
+
+import numpy as np
+import pytest
+
+import serving as S
+import workloads as W
+
+
+class SimBackend:
+    """Each request needs a fixed number of steps; an interval costs time in
+    proportion to the nodes it updates."""
+
+    def __init__(self, clock, cost_per_node_step=1e-9):
+        self.clock = clock
+        self.cost = cost_per_node_step
+        self.live = {}
+        self.next = 0
+        self.admit_order = []
+
+    def admit(self, r):
+        h = self.next
+        self.next += 1
+        n = int(r.sample.cells * r.sample.porosity)
+        self.live[h] = [r, 0, 100 * (1 + r.rid % 5), n]
+        self.admit_order.append(r.rid)
+        return h, n
+
+    def launch(self, interval):
+        self.interval = interval
+        self.clock.t += self.cost * interval * sum(v[3] for v in self.live.values())
+
+    def wait(self, tol):
+        done = []
+        for h, v in self.live.items():
+            v[1] += self.interval
+            if v[1] >= v[2]:
+                done.append((h, v[1], 1, 1e-3 * v[3], None))
+        return done
+
+    def release(self, h):
+        del self.live[h]
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += dt
+
+
+def _requests(n, rate, seed=1, fp32_share=0.5):
+    arr = S.poisson_arrivals(n, rate, seed)
+    return [S.Request(i, a, s, p) for i, (a, (s, p)) in
+            enumerate(zip(arr, S.serving_samples(n, seed, fp32_share=fp32_share)))]
+
+
+def _peak_bytes(recs):
+    ev = sorted([(r.admitted, r.bytes) for r in recs if not r.rejected] +
+                [(r.finished, -r.bytes) for r in recs if not r.rejected], key=lambda e: (e[0], e[1]))
+    cur = peak = 0
+    for _, b in ev:
+        cur += b
+        peak = max(peak, cur)
+    return peak
+
+
+@pytest.mark.parametrize("rate", [10, 200])
+def test_budget_order_and_completion(rate):
+    reqs = _requests(40, rate)
+    budget = 3 * max(S.device_bytes(r.sample, r.prec) for r in reqs)
+    clk = Clock()
+    be = SimBackend(clk)
+    recs = S.serve(reqs, be, budget, 100, 1e-6, clock=clk, sleep=clk.sleep)
+    assert all(not r.rejected and r.finished >= r.admitted >= r.arrival for r in recs)
+    assert _peak_bytes(recs) <= budget
+    # admission in arrival order (the head of the queue waits for room)
+    assert be.admit_order == [r.rid for r in sorted(reqs, key=lambda r: r.arrival)]
+    assert [r.steps for r in recs] == [100 * (1 + r.rid % 5) for r in reqs]
+
+
+def test_continuous_batching_overlaps_requests():
+    # with room for everything, a request arriving while others run is
+    # admitted at the next interval boundary instead of waiting for them
+    reqs = _requests(20, 100)
+    clk = Clock()
+    recs = S.serve(reqs, SimBackend(clk, 1e-11), 1 << 50, 100, 1e-6, clock=clk, sleep=clk.sleep)
+    overl = sum(1 for a in recs for b in recs if a.rid < b.rid and a.admitted < b.finished and b.admitted < a.finished)
+    assert overl > 0
+    assert max(r.admitted - r.arrival for r in recs) < 0.5
+
+
+def test_oversized_request_is_rejected():
+    reqs = _requests(5, 50)
+    budget = min(S.device_bytes(r.sample, r.prec) for r in reqs)
+    clk = Clock()
+    recs = S.serve(reqs, SimBackend(clk), budget, 100, 1e-6, clock=clk, sleep=clk.sleep)
+    for r, q in zip(recs, reqs):
+        assert r.rejected == (S.device_bytes(q.sample, q.prec) > budget)
+    assert sum(not r.rejected for r in recs) >= 1
+
+
+def test_fifo_latencies_and_summary():
+    fin = S.fifo_latencies([0.0, 0.5, 0.6, 5.0], [1.0, 1.0, 2.0, 0.5])
+    assert fin == [1.0, 2.0, 4.0, 5.5]
+    recs = [S.Record(i, "fp64", a, 0.0, f) for i, (a, f) in enumerate(zip([0.0, 0.5, 0.6, 5.0], fin))]
+    s = S.summarize(recs)
+    assert s["makespan_s"] == 5.5 and s["p50_latency_s"] == pytest.approx(W.percentile([1, 1.5, 3.4, 0.5], 50))
+
+
+def test_generators():
+    a = W.batch_samples(200, 7)
+    assert a == W.batch_samples(200, 7)
+    for s in a:
+        assert all(32 <= e <= 128 for e in s.dims) and 0.5 <= s.porosity <= 0.9
+        assert 3 <= s.radius <= 8 and 0.8 <= s.omega <= 1.6
+    logs = np.log([e for s in a for e in s.dims])
+    assert abs(logs.mean() - (math.log(32) + math.log(128)) / 2) < 0.1    # log-uniform
+    arr = S.poisson_arrivals(4000, 50.0, 3)
+    assert np.all(np.diff(arr) > 0) and abs(np.mean(np.diff(arr)) - 1 / 50) < 0.002
+    ss = S.serving_samples(300, 5)
+    assert all(32 <= s.dims[0] <= 192 and s.dims[0] == s.dims[2] for s, _ in ss)
+    assert 0.35 < np.mean([p == "fp32s" for _, p in ss]) < 0.65
+
+
+def test_permeability_of_plane_poiseuille(oracle):
+    # plates at z = 0 and z = nz-1 (solid), periodic x and y: the steady TRT
+    # solution with Lambda = 3/16 is the parabola u(z) = F/(2 nu) z'(h - z'),
+    # z' = z - 1/2 from the half-way wall, h fluid layers
+    nx, ny, nz = 4, 4, 10
+    mask = np.ones((nz, ny, nx), np.int32)
+    mask[0], mask[-1] = 0, 0
+    omega, F = 1.0, 1e-5
+    d, _ = oracle.list_run(mask.reshape(-1), (nx, ny, nz), (1, 1, 0), 3000, omega, F, magic=3 / 16)
+    jx = float((d @ oracle.CX.astype(float)).sum())
+    h = nz - 2
+    zp = np.arange(h) + 0.5
+    k_exact = 0.5 * float((zp * (h - zp)).sum()) / nz
+    k = W.permeability(jx, d.shape[0], nx * ny * nz, omega, F)
+    assert k == pytest.approx(k_exact, rel=1e-6)
