@@ -155,6 +155,16 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const uint32_t * __restric
 	}
 }
 
+// f = w_d in every fluid slot of an FP64 lattice (the local layout).
+__global__ __launch_bounds__(256) void k_fill_weights(double * __restrict__ pdf, uint32_t nFluid, uint32_t nCells)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= 19ull * nFluid) return;
+	const uint32_t d = (uint32_t)(i / nFluid), n = (uint32_t)(i % nFluid);
+	const double w = d == dC ? 1.0 / 3.0 : (cx(d) * cx(d) + cy(d) * cy(d) + cz(d) * cz(d) == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+	pdf[(size_t)d * nCells + n] = w;
+}
+
 // ---------------------------------------------------------------- host
 struct BatchEntry {
 	KernelData * kd = nullptr;
@@ -583,6 +593,8 @@ LbmHipBatch * LbmHipBatchCreate(const Parameters * params)
 	Batch * b = new (std::nothrow) Batch();
 	if (!b) { fail("allocating the batch failed."); return nullptr; }
 	const int n = params ? params->nKernelArgs : 0;
+	b->words.emplace_back("-shadow");        // host copies on demand (LbmHipBatchSync)
+	b->words.emplace_back("lazy");
 	for (int i = 0; i < n; ++i) b->words.emplace_back(params->KernelArgs[i]);
 	b->words.emplace_back("-quiet");
 	for (std::string & w : b->words) b->argv.push_back(&w[0]);
@@ -633,21 +645,28 @@ void LbmHipBatchDestroy(LbmHipBatch * handle)
 
 int LbmHipBatchAdd(LbmHipBatch * handle, LatticeDesc * ld, double omega, double xForce, int maxSteps)
 {
+	// allowed while an interval runs: the new domain joins at the next one
 	Batch * b = as_batch(handle);
 	if (!b || !ld) { fail("LbmHipBatchAdd: invalid arguments."); return -1; }
-	if (!idle(b, "LbmHipBatchAdd")) return -1;
 	if (!(omega > 0.0 && omega < 2.0)) { fail("omega must be in (0, 2), got %f.", omega); return -1; }
 	KernelData * kd = nullptr;
 	D3Q19ListAaPvInit_PushSoAHip(ld, &kd, &b->params);
 	if (!kd) return -1;
 	ListData * L = list_data(kd);
-	// f = w (rho 1, u 0): zero bits for shifted storage, the weights for FP64
-	if (b->args.prec == PREC_FP64) {
-		double * h = (double *)L->hPdf;
-		for (int d = 0; d < 19; ++d)
-			for (int n = 0; n < L->nFluid; ++n) h[(size_t)d * L->nCells + n] = kW[d];
-		L->c.shadowDirty = 1;
+	// f = w (rho 1, u 0) on the device: zero bits for shifted storage (as
+	// Init leaves them), the weights for FP64
+	if (b->args.prec == PREC_FP64 && L->nFluid > 0) {
+		const uint64_t total = 19ull * (uint64_t)L->nFluid;
+		(void)hipGetLastError();
+		k_fill_weights<<<(unsigned)((total + 255) / 256), 256, 0, L->c.stream>>>((double *)L->dPdf, (uint32_t)L->nFluid,
+		                                                                         (uint32_t)L->nCells);
+		if (!hip_ok(hipGetLastError(), "weight fill") || !hip_ok(hipStreamSynchronize(L->c.stream), "weight fill")) {
+			D3Q19ListAaPvDeinit_PushSoAHip(nullptr, &kd);
+			return -1;
+		}
 	}
+	L->c.shadowDirty = 0;
+	L->c.shadowStale = 1;          // the device holds the state; the host copy follows on LbmHipBatchSync
 	int id = -1;
 	for (size_t i = 0; i < b->e.size(); ++i)
 		if (b->e[i].state == 0) { id = (int)i; break; }
@@ -793,17 +812,25 @@ int LbmHipBatchStats(LbmHipBatch * handle, double * gpuSeconds, double * flups)
 
 // Times `pairs` step pairs of every candidate launch shape on the active
 // domains and keeps the fastest even and odd shapes.  The lattices are
-// stepped while tuning, so they are restored from their host copies
-// afterwards (downloaded first where the device was newer).
+// stepped while tuning, so each is copied aside on the device first and
+// copied back afterwards.
 int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out)
 {
 	Batch * b = as_batch(handle);
 	if (!b || pairs <= 0) { fail("LbmHipBatchAutotune: invalid arguments."); return -1; }
 	if (!idle(b, "LbmHipBatchAutotune")) return -1;
-	for (BatchEntry & x : b->e) {
+	std::vector<void *> backup(b->e.size(), nullptr);
+	auto release_backup = [&]() { for (void * p : backup) (void)hipFree(p); };
+	for (size_t i = 0; i < b->e.size(); ++i) {
+		BatchEntry & x = b->e[i];
 		if (x.state != 1) continue;
-		if (x.L->c.shadowDirty) { if (!list_upload(x.L)) return -1; }
-		else if (x.L->c.shadowStale && !list_download(x.L)) return -1;
+		if (!list_upload(x.L)) { release_backup(); return -1; }
+		const size_t bytes = elem_bytes(b->args.prec) * 19 * (size_t)x.L->nCells;
+		if (!hip_ok(hipMalloc(&backup[i], bytes), "allocating the tuning backup") ||
+		    !hip_ok(hipMemcpyAsync(backup[i], x.L->dPdf, bytes, hipMemcpyDeviceToDevice, b->stream), "tuning backup")) {
+			release_backup();
+			return -1;
+		}
 	}
 	const LbmHipBatchTuning saved = b->tune;
 	const int vm = b->args.evenVec ? vec_max(b->args.prec) : 1;
@@ -878,8 +905,14 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 	b->tablesDirty = true;
 	b->graphDirty = true;
 	// restore the lattices
-	for (BatchEntry & x : b->e)
-		if (x.state == 1) { x.L->c.shadowDirty = 1; x.L->c.shadowStale = 0; }
+	for (size_t i = 0; i < b->e.size(); ++i) {
+		if (!backup[i]) continue;
+		const size_t bytes = elem_bytes(b->args.prec) * 19 * (size_t)b->e[i].L->nCells;
+		if (!hip_ok(hipMemcpyAsync(b->e[i].L->dPdf, backup[i], bytes, hipMemcpyDeviceToDevice, b->stream), "tuning restore"))
+			ok = false;
+	}
+	if (!hip_ok(hipStreamSynchronize(b->stream), "tuning restore")) ok = false;
+	release_backup();
 	if (out) *out = b->tune;
 	return ok ? 0 : -1;
 }

@@ -384,6 +384,7 @@ static void shadow_set(ListData * L, size_t i, int d, double v)
 static bool list_node(const ListData * L, int x, int y, int z, uint32_t * n)
 {
 	if (!list_local_layout(L)) return false;
+	if (!L->hPdf) return fail("list-aa-pv-soa: no host copy of the lattice yet (-shadow lazy); call LbmHipSyncShadow first.");
 	if (x < 0 || y < 0 || z < 0 || x >= L->dims[0] || y >= L->dims[1] || z >= L->dims[2])
 		return fail("list-aa-pv-soa: node (%d, %d, %d) outside the lattice.", x, y, z);
 	*n = L->hGrid[((size_t)z * L->dims[1] + y) * L->dims[0] + x];
@@ -438,9 +439,22 @@ bool list_upload(ListData * L)
 	return true;
 }
 
+// The host copy, allocated here on first use with -shadow lazy.
+static bool ensure_shadow(ListData * L)
+{
+	if (L->hPdf) return true;
+	const size_t bytes = elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells;
+	L->hPdf = host_alloc(bytes);
+	if (!L->hPdf) return fail("allocating host lattice failed: %zu bytes.", bytes);
+	L->kd.Pdfs[0] = (PdfT *)L->hPdf;
+	L->kd.PdfsActive = (PdfT *)L->hPdf;
+	return true;
+}
+
 bool list_download(ListData * L)
 {
 	if (L->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
+	if (!ensure_shadow(L)) return false;
 	if (!hip_ok(hipMemcpyAsync(L->hPdf, L->dPdf, elem_bytes(L->c.args.prec) * 19 * (size_t)L->nCells,
 	                           hipMemcpyDeviceToHost, L->c.stream), "download of the device lattice")) return false;
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "hipStreamSynchronize")) return false;
@@ -818,9 +832,11 @@ void D3Q19ListAaPvInit_PushSoAHip(LatticeDesc * ld, KernelData ** kernelData, Pa
 	if (!hip_ok(hipMemsetAsync(L->dPdf, 0, pdfBytes, L->c.stream), "hipMemset")) { list_release(L); return; }
 	L->momentBlocks = (L->nFluid + 255) / 256;
 	if (!dev_alloc((void **)&L->dMoments, sizeof(double) * (2 * (size_t)L->momentBlocks + 2), "moment scratch")) { list_release(L); return; }
-	L->hPdf = host_alloc(pdfBytes);
-	if (!L->hPdf) { fail("allocating host lattice failed: %zu bytes.", pdfBytes); list_release(L); return; }
-	memset(L->hPdf, 0, pdfBytes);
+	if (!L->c.args.lazyShadow) {
+		L->hPdf = host_alloc(pdfBytes);
+		if (!L->hPdf) { fail("allocating host lattice failed: %zu bytes.", pdfBytes); list_release(L); return; }
+		memset(L->hPdf, 0, pdfBytes);
+	}
 	if (!hip_ok(hipStreamSynchronize(L->c.stream), "init")) { list_release(L); return; }
 
 	const double even = list_even_bytes(L), odd = list_odd_bytes(L);

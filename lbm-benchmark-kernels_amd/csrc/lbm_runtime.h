@@ -47,6 +47,7 @@ struct KernelArgs {
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
 	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
 	int quiet = 0;         // no informational lines at Init
+	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 };
 
 // PDF storage of the list kernel.

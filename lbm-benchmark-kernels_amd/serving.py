@@ -99,6 +99,11 @@ def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.
         now = clock() - t0
         while pending and pending[0].arrival <= now:
             queue.append(pending.popleft())
+        # the next interval of the admitted lattices runs on the device while
+        # the host builds the lattices of newly admitted requests
+        running = bool(live)
+        if running:
+            backend.launch(interval)
         while queue:                            # arrival order; the head waits for room
             r = queue[0]
             need = device_bytes(r.sample, r.prec)
@@ -115,11 +120,10 @@ def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.
             live[h] = (r, need)
             rec = recs[r.rid]
             rec.admitted, rec.bytes, rec.n_fluid = clock() - t0, need, n_fluid
-        if not live:
-            if pending:
+        if not running:
+            if not live and pending:
                 sleep(max(0.0, pending[0].arrival - (clock() - t0)))
             continue
-        backend.launch(interval)
         for h, steps, converged, momentum_x, nodes in backend.wait(tol):
             r, need = live.pop(h)
             rec = recs[r.rid]
@@ -153,16 +157,16 @@ class GpuBackend:
         return h, geo.n_fluid
 
     def launch(self, interval):
-        self.running = [p for p in PRECS if any(q == p for q, _ in self.where.values())]
-        for p in self.running:
+        self.running = {h: v for h, v in self.where.items()}
+        for p in {q for q, _ in self.running.values()}:
             self.batches[p].launch(interval)
 
     def wait(self, tol):
         done = []
-        for p in self.running:
+        for p in {q for q, _ in self.running.values()}:
             b = self.batches[p]
             b.wait(tol)
-            for h, (q, i) in list(self.where.items()):
+            for h, (q, i) in self.running.items():
                 if q != p:
                     continue
                 st = b.status(i)

@@ -123,8 +123,8 @@ def test_batch_setnode_guard(lbm):
         vals = (lbm.PdfT * 19)(*([0.05] * 19))
         lbm.hip_lib().LbmHipClearError()
         x, y, z = [int(v) for v in np.argwhere(g.mask != 0)[0][::-1]]
-        kd.contents.SetNode(kd, x, y, z, vals)     # host copy is stale: refused
-        assert "newer than the host copy" in lbm.hip_lib().LbmHipLastError().decode()
+        kd.contents.SetNode(kd, x, y, z, vals)     # no current host copy: refused
+        assert "call LbmHipSyncShadow first" in lbm.hip_lib().LbmHipLastError().decode()
         b.sync(i)
         lbm.hip_lib().LbmHipClearError()
         kd.contents.SetNode(kd, x, y, z, vals)

@@ -31,11 +31,13 @@ class SimBackend:
 
     def launch(self, interval):
         self.interval = interval
+        self.running = list(self.live)
         self.clock.t += self.cost * interval * sum(v[3] for v in self.live.values())
 
     def wait(self, tol):
         done = []
-        for h, v in self.live.items():
+        for h in self.running:
+            v = self.live[h]
             v[1] += self.interval
             if v[1] >= v[2]:
                 done.append((h, v[1], 1, 1e-3 * v[3], None))
