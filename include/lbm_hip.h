@@ -38,6 +38,14 @@
  *                     a Kernel() call, at the store).
  *   -fp16-scale <2^k> list only: power-of-two factor of the FP16 deviations
  *                     (default 4096)
+ *   -statistics       list only: call the program's KernelStatistics during the
+ *                     run as the reference's STATISTICS build does; with
+ *                     CaseData.VtkOutput set the program's VtkWrite is called
+ *                     at the reference's points as well
+ *                     (src/BenchKernelD3Q19ListAaPv.c:67-156)
+ *   -shadow eager|lazy list only: allocate the host copy at Init (default) or
+ *                     at the first LbmHipSyncShadow / download
+ *   -quiet            no informational lines at Init
  *   -vec auto|1       list only, fp32s/fp16s: the even step takes 2 (FP32)
  *                     or 4 (FP16) nodes per thread with 8-byte accesses when
  *                     the padded node count allows (auto, default), or one

@@ -44,6 +44,10 @@ void KernelVerifiy(LatticeDesc * ld, KernelData * kd, CaseData * cd, PdfT * erro
  * kernels apply it inside Kernel() themselves. */
 void KernelAddBodyForce(KernelData * kd, LatticeDesc * ld, CaseData * cd);
 
+/* src/Vtk.h:31 -- binary legacy-VTK snapshot vtk/file-<iteration>.vtk of
+ * cell types, density and velocity (the reference's format byte for byte). */
+void VtkWrite(LatticeDesc * ld, KernelData * kd, CaseData * cd, int iteration);
+
 /* Geometry (src/Geometry.h:33-47) plus a seeded porous medium. */
 typedef enum GEO_TYPES_ {
 	GEO_TYPE_MIN = 0,

@@ -286,6 +286,12 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	(void)ld;
 	AaData * A = as_aa(kd);
 	Common & c = A->c;
+	if (cd->VtkOutput) {
+		// the reference's aa-soa writes some snapshots between the even and
+		// the odd step, in the swapped layout (src/BenchKernelD3Q19Aa.c:319-331)
+		fail("aa-soa: VTK output during the run is built for list-aa-pv-soa only.");
+		return;
+	}
 	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
 	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { fail("omega must be in (0, 2), got %f.", cd->Omega); return; }
 

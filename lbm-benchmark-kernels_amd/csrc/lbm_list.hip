@@ -24,6 +24,13 @@
 // adjacency load is one coalesced 256-B access per wavefront.
 #include "lbm_list.h"
 
+// The program's output functions (the reference's Vtk.o / Kernel.o, or
+// liblbmhost.so); weak, so the library loads without them.
+extern "C" {
+void VtkWrite(LatticeDesc * ld, KernelData * kd, CaseData * cd, int iteration) __attribute__((weak));
+void KernelStatistics(KernelData * kd, LatticeDesc * ld, CaseData * cd, int iteration) __attribute__((weak));
+}
+
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -540,12 +547,59 @@ static void launch_odd(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, 
 	else       launch_odd_adj(c, pdf, cod, full, k, ft, fl, fs);
 }
 
-static bool list_steps(KernelData * kd, const CaseData * cd)
+// Output during the run, as the reference's VTK_OUTPUT / STATISTICS builds
+// write it from inside the kernel (src/BenchKernelD3Q19ListAaPv.c:67-77,
+// 128-141, 146-156): before the first step VtkWrite(-1) and
+// KernelStatistics(0); after the step pair starting at iteration it,
+// VtkWrite(it) when it % VtkModulus == 0 and KernelStatistics(it); at the end
+// VtkWrite(max) and KernelStatistics(max).  KernelStatistics writes only
+// when it % 10 == 0 or it % StatisticsModulus == 0 (src/Kernel.c:633-646), so
+// the lattice is copied to the host for those pairs only.  The functions are
+// the program's own (the reference's Vtk.o / Kernel.o, or liblbmhost.so).
+struct RunOutput {
+	LatticeDesc * ld = nullptr;
+	bool vtk = false, stats = false;
+	bool at(const CaseData * cd, int it) const
+	{
+		return (vtk && it % cd->VtkModulus == 0) ||
+		       (stats && (it % 10 == 0 || it % cd->StatisticsModulus == 0));
+	}
+};
+
+static bool run_output_setup(const ListData * L, LatticeDesc * ld, const CaseData * cd, RunOutput * o)
+{
+	o->ld = ld;
+	if (!ld) return true;
+	o->vtk = cd->VtkOutput != 0;
+	o->stats = L->c.args.statistics != 0;
+	if (o->vtk && !VtkWrite) return fail("VTK output requested but the program does not link VtkWrite.");
+	if (o->stats && !KernelStatistics) return fail("-statistics requested but the program does not link KernelStatistics.");
+	if (o->vtk && cd->VtkModulus <= 0) return fail("VtkModulus must be > 0.");
+	if (o->stats && cd->StatisticsModulus <= 0) return fail("StatisticsModulus must be > 0.");
+	return true;
+}
+
+// Host copy of the current state, for the output functions' GetNode.
+static bool output_sync(ListData * L)
+{
+	if (L->c.shadowDirty || (L->hPdf && !L->c.shadowStale)) return true;   // the host copy is current
+	return list_download(L);
+}
+
+static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 {
 	ListData * L = as_list(kd);
 	Common & c = L->c;
 	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
 	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { return fail("omega must be in (0, 2), got %f.", cd->Omega); }
+	RunOutput out;
+	if (!run_output_setup(L, ld, cd, &out)) return false;
+	CaseData * ccd = const_cast<CaseData *>(cd);
+	if (out.vtk || out.stats) {
+		if (!output_sync(L)) return false;
+		if (out.vtk) VtkWrite(ld, kd, ccd, -1);
+		if (out.stats) KernelStatistics(kd, ld, ccd, 0);
+	}
 	if (!list_upload(L)) return false;
 
 	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
@@ -586,16 +640,30 @@ static bool list_steps(KernelData * kd, const CaseData * cd)
 	};
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
+	bool segmentStart = true;  // the force of the previous step is in the stored values
+	bool synced = true;        // the host copy holds the state of the last launch
 	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
-		const bool last = it + 2 >= steps;
+		const bool output = out.at(cd, it);
+		synced = false;
+		const bool last = it + 2 >= steps || output;
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
-		even(force && it > 0);
+		even(force && !segmentStart);
 		if (perLaunch) {
 			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
 			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);
 		}
 		odd(force, force && last);
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
+		segmentStart = last;
+		if (output) {
+			if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
+			c.iteration = it + 1;
+			c.shadowStale = 1;
+			if (!output_sync(L)) return false;
+			synced = true;
+			if (out.vtk && it % cd->VtkModulus == 0) VtkWrite(ld, kd, ccd, it);
+			if (out.stats) KernelStatistics(kd, ld, ccd, it);
+		}
 	}
 	if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
 	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return false;
@@ -606,19 +674,20 @@ static bool list_steps(KernelData * kd, const CaseData * cd)
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
-	c.shadowStale = L->nFluid > 0 && steps > 0;
-	if (c.shadowDownload) {
+	if (!synced) c.shadowStale = 1;
+	if (c.shadowDownload || out.vtk || out.stats) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!list_download(L)) return false;
+		if (!output_sync(L)) return false;
 		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
 	}
+	if (out.vtk) VtkWrite(ld, kd, ccd, steps);
+	if (out.stats) KernelStatistics(kd, ld, ccd, steps);
 	return true;
 }
 
 static void list_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 {
-	(void)ld;
-	(void)list_steps(kd, cd);
+	(void)list_steps(kd, cd, ld);
 }
 
 ListData * list_data(KernelData * kd)
@@ -915,7 +984,7 @@ int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double t
 	while (r.steps < maxSteps) {
 		CaseData chunk = *cd;
 		chunk.MaxIterations = every < maxSteps - r.steps ? every : maxSteps - r.steps;
-		if (!list_steps(kd, &chunk)) { ok = false; break; }
+		if (!list_steps(kd, &chunk, nullptr)) { ok = false; break; }
 		r.steps += chunk.MaxIterations + (chunk.MaxIterations & 1);
 		seconds += kd->Duration;
 		if (!list_moments(L, &r.mass, &r.momentumX)) { ok = false; break; }

@@ -48,6 +48,7 @@ struct KernelArgs {
 	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
 	int quiet = 0;         // no informational lines at Init
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
+	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
 };
 
 // PDF storage of the list kernel.

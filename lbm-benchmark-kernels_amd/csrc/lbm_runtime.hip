@@ -194,6 +194,8 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "eager")) a->lazyShadow = 0;
 			else if (!strcmp(v, "lazy")) a->lazyShadow = 1;
 			else return fail("argument -shadow requires eager or lazy.");
+		} else if (listKernel && (is("-statistics") || is("--statistics"))) {
+			a->statistics = 1;
 		} else if (is("-quiet") || is("--quiet")) {
 			a->quiet = 1;
 		} else if (is("-even-cap") || is("--even-cap")) {
@@ -209,7 +211,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-shadow eager|lazy]\n");
+			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off] [-quiet]\n");
 			return fail("kernel help requested.");
 		} else {

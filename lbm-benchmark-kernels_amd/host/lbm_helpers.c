@@ -219,6 +219,7 @@ void KernelStatisticsAdv(KernelData * kd, LatticeDesc * ld, CaseData * cd, int i
 	if (!fh) { printf("ERROR: opening file ux-progress.dat failed.\n"); exit(1); }
 	if (iteration == 0) {
 		fprintf(fh, "# Average velocity in x direction of cross section in the middle (x = %d) of the geometry (NX = %d).\n", xm, nX);
+		fprintf(fh, "# Plot on terminal: gnuplot -e \"set terminal dumb; plot \\\"%s\\\";\"\n", "ux-progress.dat");
 		fprintf(fh, "# iteration, avg ux\n");
 	}
 	fprintf(fh, "%d %e\n", iteration, uSum / cnt);
@@ -228,6 +229,8 @@ void KernelStatisticsAdv(KernelData * kd, LatticeDesc * ld, CaseData * cd, int i
 	fh = fopen("density-ux.dat", "w");
 	if (!fh) { printf("ERROR: opening file density-ux.dat failed.\n"); exit(1); }
 	fprintf(fh, "# Average density and average x velocity over each cross section in x direction. Snapshot taken at iteration %d.\n", iteration);
+	fprintf(fh, "# Plot on terminal: gnuplot -e \"set terminal dumb; plot \\\"%s\\\" u 1:2; plot \\\"%s\\\" u 1:3;\"\n",
+	        "density-ux.dat", "density-ux.dat");
 	fprintf(fh, "# x, avg density, avg ux\n");
 	for (int x = 0; x < nX; ++x) {
 		double rs = 0.0, us = 0.0;

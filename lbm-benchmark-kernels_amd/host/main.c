@@ -37,7 +37,7 @@ static void usage(void)
 	printf("      [-dims XxYxZ] [-geometry box|channel|pipe|blocks[-<n>]|fluid|porous[-<r>[-<porosity>[-<seed>]]]]\n");
 	printf("      [-iterations <iterations>] [-rho-in <density>] [-rho-out <density>] [-omega <omega>]\n");
 	printf("      [-x-force <force>] [-kernel <kernel>] [-periodic-x] [-periodic-y] [-periodic-z]\n");
-	printf("      [-t <host threads>] [-verify]\n");
+	printf("      [-t <host threads>] [-verify] [-vtk [<every-n-iteration>]] [-statistics <every-n-iteration>]\n");
 	printf("      -- <kernel specific parameters>\n");
 }
 
@@ -48,6 +48,7 @@ int main(int argc, char * argv[])
 	const char * kernelName = "list-aa-pv-soa";
 	int verify = 0;
 	int nThreads = 1;
+	int statistics = 0;
 	int periodic[3] = { 0, 0, 0 };
 	CaseData cd;
 	memset(&cd, 0, sizeof(cd));
@@ -109,9 +110,19 @@ int main(int argc, char * argv[])
 		else if (IS("-periodic-x")) periodic[0] = 1;
 		else if (IS("-periodic-y")) periodic[1] = 1;
 		else if (IS("-periodic-z")) periodic[2] = 1;
-		else if (IS("-vtk") || IS("-statistics")) {
-			printf("ERROR: %s output is not part of this build.\n", a);
-			return 1;
+		else if (IS("-vtk")) {
+			/* an optional positive number is the output period (src/Main.c:266-286) */
+			cd.VtkOutput = 1;
+			if (i + 1 < argc && atoi(argv[i + 1]) > 0) cd.VtkModulus = atoi(argv[++i]);
+		}
+		else if (IS("-statistics")) {
+			NEED();
+			cd.StatisticsModulus = atoi(argv[++i]);
+			if (cd.StatisticsModulus <= 0) {
+				printf("ERROR: the iteration count for -statistics must be > 0.\n");
+				return 1;
+			}
+			statistics = 1;
 		}
 		else if (IS("-h") || IS("-help")) { usage(); return 1; }
 		else if (!strcmp(a, "--")) {
@@ -171,6 +182,22 @@ int main(int argc, char * argv[])
 	}
 	printf("#\n# - kernel:            %s\n#\n", kf->Name);
 
+	/* -statistics: the list kernel writes KernelStatistics during the run
+	 * (the reference's STATISTICS build) when given the kernel argument */
+	char ** kargs = NULL;
+	if (statistics) {
+		if (strcasecmp(kf->Name, "list-aa-pv-soa") != 0) {
+			printf("ERROR: -statistics during the run is built for list-aa-pv-soa only.\n");
+			return 1;
+		}
+		kargs = (char **)malloc(sizeof(char *) * (size_t)(p.nKernelArgs + 1));
+		if (!kargs) { printf("ERROR: allocating kernel arguments failed.\n"); return 1; }
+		for (int j = 0; j < p.nKernelArgs; ++j) kargs[j] = p.KernelArgs[j];
+		kargs[p.nKernelArgs] = (char *)"-statistics";
+		p.KernelArgs = kargs;
+		p.nKernelArgs += 1;
+	}
+
 	KernelData * kd = NULL;
 	kf->Init(&ld, &kd, &p);
 	if (kd == NULL) { printf("ERROR: kernel initialisation failed.\n"); return 1; }
@@ -197,6 +224,7 @@ int main(int argc, char * argv[])
 	const double volGiB = loopBalance * ld.nFluid * cd.MaxIterations / 1024. / 1024. / 1024.;
 
 	kf->Deinit(&ld, &kd);
+	free(kargs);
 
 	const double perf = (double)ld.nFluid * (double)cd.MaxIterations / duration / 1.e6;
 	printf("#\n# Evaluation Stats\n");

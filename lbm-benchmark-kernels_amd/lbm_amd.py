@@ -125,7 +125,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
                "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait"]
-HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity",
+HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity", "VtkWrite",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
                 "GeoCreateByStr", "GeoFree"]
@@ -147,7 +147,9 @@ def host_lib() -> C.CDLL:
         p = lib_path("liblbmhost.so")
         if not os.path.exists(p):
             raise LibraryMissing(f"{p} is not built; run __graft_entry__.build()")
-        L = C.CDLL(p)
+        # global: liblbmhip.so's kernels call the program's VtkWrite and
+        # KernelStatistics (weak references resolved when it loads)
+        L = C.CDLL(p, mode=C.RTLD_GLOBAL)
         L.GeoCreateByStr.argtypes = [C.c_char_p, C.c_int * 3, C.c_int * 3, C.POINTER(LatticeDesc)]
         L.GeoCreateByType.argtypes = [C.c_int, C.c_void_p, C.c_int * 3, C.c_int * 3, C.POINTER(LatticeDesc)]
         L.GeoFree.argtypes = [C.POINTER(LatticeDesc)]
@@ -160,6 +162,8 @@ def host_lib() -> C.CDLL:
         L.KernelDensity.restype = PdfT
         L.KernelStatisticsAdv.argtypes = [C.POINTER(KernelData), C.POINTER(LatticeDesc),
                                           C.POINTER(CaseData), C.c_int, C.c_int]
+        L.KernelStatistics.argtypes = [C.POINTER(KernelData), C.POINTER(LatticeDesc), C.POINTER(CaseData), C.c_int]
+        L.VtkWrite.argtypes = [C.POINTER(LatticeDesc), C.POINTER(KernelData), C.POINTER(CaseData), C.c_int]
         _libs["host"] = L
     return _libs["host"]
 
@@ -169,6 +173,7 @@ def hip_lib() -> C.CDLL:
         p = lib_path("liblbmhip.so")
         if not os.path.exists(p):
             raise LibraryMissing(f"{p} is not built; run __graft_entry__.build()")
+        host_lib()                      # first: provides the output functions
         L = C.CDLL(p)
         KD = C.POINTER(KernelData)
         L.LbmHipSetErrorMode.argtypes = [C.c_int]

@@ -303,7 +303,9 @@ class Reference:
         path = os.path.join(HERE, "_ref", f"libref_{variant}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(path)
-        L = self.lib = C.CDLL(path)
+        # the reference's internal calls must reach its own functions, never
+        # same-named ones of a library loaded before (liblbmhost.so)
+        L = self.lib = C.CDLL(path, mode=os.RTLD_DEEPBIND)
         L.ref_list_run.argtypes = [_i32p, C.c_int * 3, C.c_int * 3, C.c_int, C.c_int, C.c_void_p,
                                    C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -313,6 +315,7 @@ class Reference:
                                  C.c_void_p, C.POINTER(_RefResult)]
         L.ref_geometry.argtypes = [C.c_char_p, C.c_int * 3, C.c_int * 3, _i32p, C.c_int * 3]
         L.ref_pad_cells.argtypes = [C.c_int, C.c_char_p]
+        L.ref_set_output.argtypes = [C.c_int, C.c_int, C.c_int]
 
     @staticmethod
     def available(variant: str = "verif") -> bool:

@@ -58,6 +58,11 @@ int ref_aa_run(const int * mask, const int dims[3], const int periodic[3],
 int ref_geometry(const char * type, const int dims[3], const int periodic[3],
                  int * mask, int ldPeriodic[3]);
 
+/* CaseData.VtkOutput / VtkModulus / StatisticsModulus of the following runs
+ * (VTK and statistics files appear only with libref_io.so, in the working
+ * directory). */
+void ref_set_output(int vtkOutput, int vtkModulus, int statisticsModulus);
+
 /* PadCells(nCells, sizeof(double), PadInfoFromStr(str)); str NULL = default. */
 int ref_pad_cells(int nCells, const char * str);
 

@@ -38,6 +38,15 @@ void refh_make_lattice(LatticeDesc * ld, const int * mask, const int dims[3], co
 	}
 }
 
+static int g_vtkOutput = 0, g_vtkModulus = 100, g_statisticsModulus = 100;
+
+void ref_set_output(int vtkOutput, int vtkModulus, int statisticsModulus)
+{
+	g_vtkOutput = vtkOutput;
+	g_vtkModulus = vtkModulus;
+	g_statisticsModulus = statisticsModulus;
+}
+
 void refh_case(CaseData * cd, double omega, double xForce, int iterations)
 {
 	memset(cd, 0, sizeof(*cd));
@@ -46,9 +55,9 @@ void refh_case(CaseData * cd, double omega, double xForce, int iterations)
 	cd->RhoOut = 1.0;
 	cd->XForce = xForce;
 	cd->MaxIterations = iterations;
-	cd->VtkOutput = 0;
-	cd->VtkModulus = 100;
-	cd->StatisticsModulus = 100;
+	cd->VtkOutput = g_vtkOutput;
+	cd->VtkModulus = g_vtkModulus;
+	cd->StatisticsModulus = g_statisticsModulus;
 }
 
 void refh_threads(int nThreads)

@@ -94,7 +94,8 @@ def test_symbol_lists_in_mirror(lbm):
     assert set(lbm.HOST_SYMBOLS) <= _exported("liblbmhost.so")
     # the HIP library must not clash with the reference's Kernel.o/Geometry.o
     # when both are linked into the reference driver
-    clash = {"KernelDensity", "KernelVerifiy", "GeoCreateByStr", "PadCells", "D3Q19_X", "MemAlloc"}
+    clash = {"KernelDensity", "KernelVerifiy", "GeoCreateByStr", "PadCells", "D3Q19_X", "MemAlloc", "VtkWrite",
+             "KernelStatistics"}
     assert not (clash & _exported("liblbmhip.so"))
 
 

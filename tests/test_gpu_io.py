@@ -1,0 +1,87 @@
+"""GPU: the list kernel's run-time output (VTK snapshots and statistics files
+at the reference's points of a VTK_OUTPUT / STATISTICS build) equals, byte
+for byte, the same output functions replayed on the CPU over the oracle's
+states at those points.  (tests/test_io.py shows the functions themselves
+write what the reference's write.)"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from test_io import node_view_kd, write_outputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _in_dir(path, fn):
+    old = os.getcwd()
+    os.makedirs(path, exist_ok=True)
+    os.chdir(path)
+    try:
+        return fn()
+    finally:
+        os.chdir(old)
+
+
+def _files(where):
+    out = {}
+    for root, _, files in os.walk(where):
+        for f in files:
+            p = os.path.join(root, f)
+            out[os.path.relpath(p, where)] = open(p, "rb").read()
+    return out
+
+
+def test_run_time_output_matches_replay(lbm, oracle, tmp_path):
+    geo = lbm.Geometry.from_string("porous-2-0.6-3", (16, 14, 14), (1, 0, 0))
+    steps, vmod, smod, omega, F = 24, 4, 6, 1.1, 2e-5
+    H = lbm.host_lib()
+
+    def gpu():
+        with lbm.Kernel("list-aa-pv-soa", geo, ["-statistics"]) as k:
+            k.set_initial_weights()
+            cd = lbm.case_data(omega=omega, xforce=F, iterations=steps)
+            cd.VtkOutput, cd.VtkModulus, cd.StatisticsModulus = 1, vmod, smod
+            k.run(cd)
+    _in_dir(str(tmp_path / "gpu"), gpu)
+    got = _files(str(tmp_path / "gpu"))
+
+    # replay: the reference's call sequence over the oracle's states
+    m = geo.mask.reshape(-1)
+    state = {}
+
+    def at(s):
+        if s not in state:
+            state[s] = oracle.list_run(m, geo.dims, geo.periodic, s, omega, F, vsize=1)[0]
+        return state[s]
+    calls = [(0, "vtk", -1), (0, "stats", 0)]
+    for it in range(0, steps, 2):
+        if it % vmod == 0:
+            calls.append((it + 2, "vtk", it))
+        if it % 10 == 0 or it % smod == 0:
+            calls.append((it + 2, "stats", it))
+    calls += [(steps, "vtk", steps), (steps, "stats", steps)]
+    cd = lbm.case_data(omega=omega, xforce=F, iterations=steps)
+    cd.VtkOutput, cd.VtkModulus, cd.StatisticsModulus = 1, vmod, smod
+    for fn in (H.VtkWrite, H.KernelStatistics):
+        fn.argtypes = [C.c_void_p] * 3 + [C.c_int]
+    where = str(tmp_path / "replay")
+    for s, kind, it in calls:
+        kd, keep = node_view_kd(lbm, geo, at(s))
+        write_outputs(H.VtkWrite, H.KernelStatistics, lbm, geo, kd, cd, [(kind, it)], where)
+        del keep
+    want = _files(where)
+    assert sorted(got) == sorted(want)
+    assert "vtk/file-0012.vtk" in got and "vtk/file-0024.vtk" in got
+    for name in want:
+        assert got[name] == want[name], name
+
+
+def test_aa_rejects_run_time_vtk(lbm):
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with lbm.Kernel("aa-soa", geo) as k:
+        cd = lbm.case_data(iterations=2)
+        cd.VtkOutput = 1
+        with pytest.raises(RuntimeError, match="list-aa-pv-soa only"):
+            k.run(cd)
