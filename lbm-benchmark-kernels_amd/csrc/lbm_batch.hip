@@ -77,8 +77,12 @@ __global__ __launch_bounds__(512) void k_batch_odd(const BatchDom<typename Codec
 {
 	using R = typename Codec::R;
 	using S = typename Codec::S;
-	const uint32_t nc = *count;
-	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+	// the grid is a multiple of 8 * kXcdGroup blocks: each round of
+	// gridDim.x chunks is handed out in the XCD-grouped order
+	const uint32_t nc = *count, slot = xcd_grouped_block(blockIdx.x);
+	for (uint32_t base = 0; base < nc; base += gridDim.x) {
+		const uint32_t c = base + slot;
+		if (c >= nc) break;
 		const Chunk ch = chunks[c];
 		const BatchDom<R> & D = doms[ch.dom];
 		const uint32_t n = ch.node0 + threadIdx.x;
@@ -359,7 +363,9 @@ static bool rebuild_tables(Batch * b)
 	if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&oddPerCU, oddFn, b->tune.oddBlock, 0), "occupancy query")) return false;
 	const uint32_t og = oddPerCU > 0 ? (uint32_t)(oddPerCU * b->nCU) : og0;
 	const uint32_t ne = counts[0] ? counts[0] : 1, no = counts[1] ? counts[1] : 1;
-	const uint32_t newEven = eg < ne ? eg : ne, newOdd = og < no ? og : no;
+	const uint32_t groupBlocks = 8 * kXcdGroup;
+	const uint32_t newEven = eg < ne ? eg : ne;
+	const uint32_t newOdd = ((og < no ? og : no) + groupBlocks - 1) / groupBlocks * groupBlocks;
 	// the graph keeps its grid: a shrinking chunk list needs no new capture
 	if (newEven > b->evenGrid || newOdd > b->oddGrid) b->graphDirty = true;
 	if (b->graphDirty) {

@@ -279,7 +279,7 @@ template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, class Adj>
 __global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec cod, Adj adjacency, uint32_t nFluid,
                                                   uint32_t nCells, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
-	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t n = xcd_grouped_block(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	uint32_t a[18];
 	adjacency.fetch(n, a);
@@ -609,7 +609,8 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	if (perLaunch && !common_events(&c, steps)) return false;
 
 	const int bs = c.args.blockSize;
-	const dim3 grid((unsigned)((L->nFluid + bs - 1) / bs)), block((unsigned)bs);
+	const unsigned oddBlocks = (unsigned)((L->nFluid + bs - 1) / bs), groupBlocks = 8 * kXcdGroup;
+	const dim3 grid((oddBlocks + groupBlocks - 1) / groupBlocks * groupBlocks), block((unsigned)bs);
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
