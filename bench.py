@@ -270,6 +270,30 @@ def serve_mode(args, lbm):
                    {"p99_latency_s": top["served"]["p99_latency_s"], "serving": out})
 
 
+def aa_mode(args, lbm):
+    """aa-soa (full-array AA kernel) on the duct channel: 64^3 x 1000 steps
+    (the small configuration the reference also runs on its CPU) and 256^3."""
+    out = {}
+    for edge, steps in ((64, 1000), (256, 200)):
+        geo = lbm.Geometry.from_string("channel", (edge, edge, edge), (1, 0, 0))
+        with lbm.Kernel("aa-soa", geo, ["-lambda", repr(LAMBDA), "-quiet"] + args.kernel_args.split()) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            k.run(lbm.case_data(omega=1.0, xforce=1e-5, iterations=20))
+            k.run(lbm.case_data(omega=1.0, xforce=1e-5, iterations=steps))
+            t = k.timings()
+        nf = geo.n_fluid
+        b = 2 * 19 * 8 + 4          # PDFs plus the cell's info word
+        out[f"{edge}^3"] = {"n_fluid": nf, "steps": steps, "mflups": round(nf * steps / (t.loopMs / 1e3) / 1e6, 1),
+                            "even_ms": round(t.evenMsAvg, 5), "odd_ms": round(t.oddMsAvg, 5),
+                            "even_gbs": round(b * nf / (t.evenMsAvg / 1e3) / 1e9, 1),
+                            "odd_gbs": round(b * nf / (t.oddMsAvg / 1e3) / 1e9, 1)}
+        _log(f"aa-soa {edge}^3: {out[f'{edge}^3']}")
+    return _common("MFLUP/s (D3Q19 TRT aa-soa FP64, 256^3 duct channel)", out["256^3"]["mflups"], "MFLUP/s", 200,
+                   "aa-soa, channel (x periodic, walls y/z), omega 1.0, Lambda 3/16, F 1e-5, FP64, f = w at start",
+                   {"dtype": "f64", "aa": out})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,10 +303,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prec", choices=["fp64", "fp32s", "fp16s"], default="fp64",
                     help="PDF storage: fp64 (the headline), or deviations f - w in FP32 / scaled FP16")
-    ap.add_argument("--mode", choices=["list", "batch", "precision", "serve"], default="list",
+    ap.add_argument("--mode", choices=["list", "batch", "precision", "serve", "aa"], default="list",
                     help="list: the 256^3 headline; batch: many porous samples in one batch; "
                          "precision: the 64^3 channel to steady state in every storage type; "
                          "serve: Poisson request streams through the continuous-batching server")
+    ap.add_argument("--kernel-args", default="", help="extra kernel arguments (list and aa modes), space separated")
     ap.add_argument("--rates", default="10,50,200,500", help="serve: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=32, help="serve: requests per rate")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget (GB)")
@@ -307,7 +332,7 @@ def main():
     import lbm_amd as lbm
 
     if args.mode != "list":
-        line = {"batch": batch_mode, "precision": precision_mode, "serve": serve_mode}[args.mode](args, lbm)
+        line = {"batch": batch_mode, "precision": precision_mode, "serve": serve_mode, "aa": aa_mode}[args.mode](args, lbm)
         os.dup2(json_fd, 1)
         print(json.dumps(line), flush=True)
         return
@@ -317,7 +342,7 @@ def main():
     geo = lbm.Geometry.from_string(GEOMETRY, DIMS)
     nf = geo.n_fluid
     print(f"bench: {GEOMETRY} {DIMS} nFluid={nf} steps={steps} warmup={warm}", file=sys.stderr, flush=True)
-    k = lbm.Kernel("list-aa-pv-soa", geo, ["-lambda", repr(LAMBDA), "-prec", args.prec])
+    k = lbm.Kernel("list-aa-pv-soa", geo, ["-lambda", repr(LAMBDA), "-prec", args.prec] + args.kernel_args.split())
     info = k.list_info()
     b_even = info.bytesPerFlupEven
     k.set_initial_weights()

@@ -22,9 +22,13 @@
  *   -bs <n>           threads per block of the step kernels (64..512, step 64)
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
- *   -nt on|off        non-temporal loads/stores in the even step (default on)
- *   -even-cap on|off  even step in 512-thread blocks, one block per compute
- *                     unit (default on; -bs then applies to the odd step)
+ *   -nt on|off        non-temporal loads/stores in the even step (default on
+ *                     for list-aa-pv-soa, off for aa-soa: measured faster)
+ *   -even-cap on|off|force
+ *                     even step in 512-thread blocks, one block per compute
+ *                     unit (default on for lattices of at least 16 such
+ *                     blocks per compute unit; force: at any size; -bs then
+ *                     applies to the odd step)
  *   -prec fp64|fp32s|fp16s
  *                     list only: PDF storage.  fp64 (default) is the
  *                     reference's PdfT = double.  fp32s/fp16s store the

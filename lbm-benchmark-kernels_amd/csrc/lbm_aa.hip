@@ -110,18 +110,20 @@ __global__ __launch_bounds__(256) void k_aa_info(const int * __restrict__ mask, 
 	if (__lane_id() == 0 && bounces) atomicAdd(nBounce, (uint32_t)bounces);
 }
 
+// Cell of this thread: z along threadIdx.x, blockDim.y rows of y per block.
 __device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int & z)
 {
 	z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-	y = (int)blockIdx.y;
+	y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
 	x = (int)blockIdx.z;
-	return z < g.Z;
+	return z < g.Z && y < g.Y;
 }
 
 // Even step: local load, collision, store of direction d into slot inv(d)
 // of the same cell (src/BenchKernelD3Q19Aa.c:146-290).
-template <bool FORCE_LOAD>
-__global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
+// NT: non-temporal loads and stores (each value is touched once per step).
+template <bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_aa_even(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
                                                  TrtC<double> k, ForceT<double> ft)
 {
 	int x, y, z;
@@ -132,12 +134,12 @@ __global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * 
 	double f[19];
 #pragma unroll
 	for (int d = 0; d < 19; ++d) {
-		f[d] = pdf[base + (uint64_t)d * g.plane];
+		f[d] = ld<NT>(&pdf[base + (uint64_t)d * g.plane]);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	trt_collide(f, k);
 #pragma unroll
-	for (int d = 0; d < 19; ++d) pdf[base + (uint64_t)inv_dir(d) * g.plane] = f[d];
+	for (int d = 0; d < 19; ++d) st<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], f[d]);
 }
 
 // Odd step: direction d arrives from the pull neighbour x - c_d, slot inv(d)
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(256) void k_aa_even(double * pdf, const uint32_t * 
 // node view the reference forces after the step, so it gets the increment
 // here too.
 template <bool FORCE_LOAD, bool FORCE_STORE>
-__global__ __launch_bounds__(256) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
+__global__ __launch_bounds__(512) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
                                                 TrtC<double> k, ForceT<double> ft)
 {
 	int x, y, z;
@@ -314,14 +316,24 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const AaGeom g = A->g;
 	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
+	// even step: blocks of 512 threads over several y rows at one block per
+	// CU, as the list kernel's even step (stream_shape)
+	const StreamShape es = stream_shape(&c, bs, (uint64_t)grid.x * bs * g.Y * g.X);
+	const unsigned rows = (unsigned)(es.block / bs);
+	const dim3 eblock((unsigned)bs, rows), egrid(grid.x, ((unsigned)g.Y + rows - 1) / rows, grid.z);
+	// non-temporal accesses measured slower here than in the list kernel: off unless asked for
+	const bool nt = c.args.nontemporal == 1;
 
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
 	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
 		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
-		if (force && it > 0) k_aa_even<true><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else                 k_aa_even<false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		const bool fl = force && it > 0;
+		if (fl && nt)  k_aa_even<true, true><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else if (fl)   k_aa_even<true, false><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else if (nt)   k_aa_even<false, true><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
+		else           k_aa_even<false, false><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
 		if (perLaunch) {
 			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
 			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);

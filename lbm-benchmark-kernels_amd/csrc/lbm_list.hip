@@ -614,11 +614,11 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
-	const StreamShape es = stream_shape(&c, bs);
-	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, c.stream, nF, nC,
-	                       c.args.adjRle != 0, c.args.nontemporal != 0};
-	const float f16s = (float)(1 << c.args.f16Shift);
 	const int vec = list_even_vec(L);
+	const StreamShape es = stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
+	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, c.stream, nF, nC,
+	                       c.args.adjRle != 0, c.args.nontemporal != 0};   // default on (-1)
+	const float f16s = (float)(1 << c.args.f16Shift);
 	auto even = [&](bool fl) {
 		switch (c.args.prec) {
 			case PREC_FP32S:

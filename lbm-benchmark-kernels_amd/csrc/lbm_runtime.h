@@ -41,8 +41,8 @@ struct KernelArgs {
 	double magic = 1.0 / 12.0;
 	int blockSize = 256;
 	int adjRle = 1;        // list: odd step reads run-compressed adjacency
-	int nontemporal = 1;   // streaming (non-temporal) loads/stores in the even step
-	int evenCap = 1;       // even step: 512-thread blocks, one block per CU
+	int nontemporal = -1;  // non-temporal loads/stores in the even step: 1 on, 0 off, -1 the kernel's default
+	int evenCap = 1;       // even step: 512-thread blocks, one block per CU (2: also on small lattices)
 	int prec = 0;          // list: PDF storage, see enum Prec
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
 	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
@@ -89,6 +89,7 @@ struct Common {
 	int device = 0;
 	int ldsPerCU = 0;           // bytes of LDS per compute unit
 	int ldsPerBlock = 0;        // largest dynamic LDS a block may request
+	int nCU = 0;                // compute units of the device
 	hipStream_t stream = nullptr;
 	KernelArgs args;
 	int iteration = -1;         // parity of the last completed step
@@ -114,16 +115,17 @@ Common * aa_common(KernelData * kd);
 
 bool common_init(Common * c, const Parameters * params, bool listKernel);
 void common_free(Common * c);
-// Launch shape of a streaming step kernel.  With the cap, blocks of 512
-// threads request just over half the CU's LDS, so one block -- 8
-// wavefronts -- runs per CU: fewer concurrent streams, measured 6.0 TB/s for
-// the even step against 5.4 TB/s at full occupancy.  The LDS is reserved,
-// never used.
+// Launch shape of a streaming step kernel over `threads` threads.  With the
+// cap, blocks of 512 threads request just over half the CU's LDS, so one
+// block -- 8 wavefronts -- runs per CU: fewer concurrent streams, measured
+// 6.0 TB/s for the even step against 5.4 TB/s at full occupancy.  The LDS is
+// reserved, never used.  Lattices too small for 16 such blocks per CU keep
+// the plain launch (measured faster at 64^3).
 struct StreamShape {
 	int block;
 	size_t lds;
 };
-StreamShape stream_shape(const Common * c, int defaultBlock);
+StreamShape stream_shape(const Common * c, int defaultBlock, uint64_t threads);
 // Grows the per-launch event pool to hold `launches` launches.
 bool common_events(Common * c, int launches);
 // Fills c->t from the recorded events (stream must be synchronised).

@@ -199,11 +199,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 		} else if (is("-quiet") || is("--quiet")) {
 			a->quiet = 1;
 		} else if (is("-even-cap") || is("--even-cap")) {
-			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on or off.");
+			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on, off or force.");
 			const char * v = p->KernelArgs[++i];
 			if (!strcmp(v, "on")) a->evenCap = 1;
 			else if (!strcmp(v, "off")) a->evenCap = 0;
-			else return fail("argument -even-cap requires on or off.");
+			else if (!strcmp(v, "force")) a->evenCap = 2;
+			else return fail("argument -even-cap requires on, off or force.");
 		} else if (is("-bs") || is("--bs")) {
 			if (!arg_int(p, i, &a->blockSize, "-bs", 64, 512)) return false;
 			if (a->blockSize % 64) return fail("argument -bs must be a multiple of 64.");
@@ -212,7 +213,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
-			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off] [-quiet]\n");
+			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			return fail("kernel help requested.");
 		} else {
 			return fail("unknown kernel parameter: %s.", s);
@@ -310,6 +311,8 @@ bool common_init(Common * c, const Parameters * params, bool listKernel)
 	            "hipDeviceGetAttribute")) return false;
 	if (!hip_ok(hipDeviceGetAttribute(&c->ldsPerBlock, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device),
 	            "hipDeviceGetAttribute")) return false;
+	if (!hip_ok(hipDeviceGetAttribute(&c->nCU, hipDeviceAttributeMultiprocessorCount, c->device),
+	            "hipDeviceGetAttribute")) return false;
 	if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
 	if (!hip_ok(hipEventCreate(&c->evLoop[0]), "hipEventCreate")) return false;
 	if (!hip_ok(hipEventCreate(&c->evLoop[1]), "hipEventCreate")) return false;
@@ -326,10 +329,11 @@ void common_free(Common * c)
 	if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
 }
 
-StreamShape stream_shape(const Common * c, int defaultBlock)
+StreamShape stream_shape(const Common * c, int defaultBlock, uint64_t threads)
 {
 	StreamShape sh = {defaultBlock, 0};
 	if (!c->args.evenCap) return sh;
+	if (c->args.evenCap != 2 && threads < 16ull * 512 * (uint64_t)(c->nCU > 0 ? c->nCU : 1)) return sh;
 	const size_t want = (size_t)c->ldsPerCU / 2 + 8192;
 	if (c->ldsPerCU <= 0 || want > (size_t)c->ldsPerBlock) return sh;
 	sh.block = 512;

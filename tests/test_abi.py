@@ -130,6 +130,7 @@ def test_init_fails_loudly_without_device(lbm):
     (["-fp16-scale", "3000"], "power of two"),
     (["-fp16-scale", "0"], "must be in"),
     (["-vec", "2"], "auto or 1"),
+    (["-even-cap", "maybe"], "on, off or force"),
 ])
 def test_kernel_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))

@@ -236,7 +236,8 @@ def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
     want, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1)
     for args in (["-adj", "rle"], ["-adj", "full"], ["-adj", "rle", "-nt", "off"], ["-adj", "full", "-nt", "off"],
                  ["-adj", "rle", "-bs", "128"], ["-adj", "rle", "-bs", "64", "-blk", "5"],
-                 ["-adj", "rle", "-even-cap", "off"], ["-adj", "full", "-even-cap", "off", "-bs", "512"]):
+                 ["-adj", "rle", "-even-cap", "off"], ["-adj", "full", "-even-cap", "off", "-bs", "512"],
+                 ["-adj", "rle", "-even-cap", "force"], ["-adj", "full", "-even-cap", "force", "-nt", "off"]):
         with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
             info = k.list_info()
             assert info.adjRle == (args[1] == "rle")
@@ -249,3 +250,28 @@ def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
             assert np.array_equal(got, w2), args
         else:
             assert np.array_equal(got, want), args
+
+
+@pytest.mark.parametrize("args", [[], ["-nt", "on"], ["-even-cap", "force"], ["-nt", "on", "-even-cap", "force"],
+                                  ["-even-cap", "off"]])
+def test_aa_streaming_options(lbm, oracle, args):
+    # the even step's launch shape (one 512-thread block over several y rows
+    # per CU) and non-temporal accesses change no bit; odd Y and Z sizes
+    geo = lbm.Geometry.from_string("porous-3-0.75-9", (40, 37, 70))
+    m = geo.mask.reshape(-1)
+    want, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1)
+    got, _, _ = _run(lbm, "aa-soa", geo, 30, 1.25, 3e-5, args)
+    assert np.array_equal(got, want), args
+
+
+@pytest.mark.parametrize("name", ["list-aa-pv-soa", "aa-soa"])
+def test_large_lattice_default_launch(lbm, oracle, name):
+    # above the size where the even step runs one 512-thread block per CU
+    geo = lbm.Geometry.from_string("porous-4-0.85-5", (176, 128, 128))
+    assert geo.n_fluid > 16 * 512 * 256
+    m = geo.mask.reshape(-1)
+    print(f"{name} 176x128x128 nFluid={geo.n_fluid}", flush=True)
+    want, _ = oracle.list_run(m, geo.dims, geo.periodic, 10, 1.3, 2e-5, vsize=1)
+    for args in ([], ["-even-cap", "off"]):
+        got, _, _ = _run(lbm, name, geo, 10, 1.3, 2e-5, args)
+        assert np.array_equal(got, want), args

@@ -63,7 +63,8 @@ def test_shifted_options_and_split_calls(lbm, oracle, prec):
     want, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, runs, 1.25, 3e-5, prec, magic=LAM)
     for extra in ([], ["-adj", "full"], ["-nt", "off"], ["-even-cap", "off", "-bs", "128"],
                   ["-bs", "64", "-blk", "5"], ["-force", "on", "-adj", "full", "-nt", "off"],
-                  ["-vec", "1"], ["-pad", "no"], ["-pad", "no", "-even-cap", "off", "-nt", "off"]):
+                  ["-vec", "1"], ["-pad", "no"], ["-pad", "no", "-even-cap", "off", "-nt", "off"],
+                  ["-even-cap", "force"], ["-even-cap", "force", "-pad", "no"]):
         print(f"porous48 -prec {prec} {extra}", flush=True)
         got, _ = _gpu(lbm, geo, runs, 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
         assert np.array_equal(got, want), extra
