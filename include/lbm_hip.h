@@ -49,6 +49,11 @@
  *                     (src/BenchKernelD3Q19ListAaPv.c:67-156)
  *   -shadow eager|lazy list only: allocate the host copy at Init (default) or
  *                     at the first LbmHipSyncShadow / download
+ *   -odd-order band|group|plain
+ *                     list only: order of the odd step's blocks over the
+ *                     device's 8 XCDs -- band (default): each XCD takes the
+ *                     blocks of one eighth of the y range; group: 8
+ *                     consecutive blocks per XCD in turn; plain: launch order
  *   -quiet            no informational lines at Init
  *   -vec auto|1       list only, fp32s/fp16s: the even step takes 2 (FP32)
  *                     or 4 (FP16) nodes per thread with 8-byte accesses when

@@ -29,6 +29,8 @@ struct ListData {
 	void * dPdf = nullptr;       // 19 * nCells values of the storage type
 	void * hPdf = nullptr;       // host shadow, same layout and type, == kd.Pdfs[0]
 	double * dMoments = nullptr; // reduction scratch: 2 per block + 2
+	uint32_t * dOddOrder = nullptr; // odd step, -odd-order band: node block of each launched block
+	uint32_t oddGrid = 0;           // blocks launched for the odd step
 	int momentBlocks = 0;
 	uint32_t * hGrid = nullptr;  // cell -> node id, for GetNode/SetNode
 };
@@ -166,6 +168,18 @@ __device__ __forceinline__ void even_nodes(typename Codec::S * pdf, Codec cod, u
 // adds the force increment of this step to each stored value: a value
 // stored through a bounce-back entry is read back by the same node as
 // direction inv(d), any other by the neighbour as d.
+// Node block of launched block b in the odd step: in launch order (plain),
+// XCD-grouped, or from a table (band: every XCD takes the blocks of one
+// eighth of the y range, so neighbours in x and y mostly meet in its L2).
+struct OddOrder {
+	const uint32_t * table;    // band order, or nullptr
+	uint32_t grouped;          // 1: xcd_grouped_block
+	__device__ __forceinline__ uint32_t block(uint32_t b) const
+	{
+		return table ? table[b] : grouped ? xcd_grouped_block(b) : b;
+	}
+};
+
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE>
 __device__ __forceinline__ void odd_node(typename Codec::S * pdf, Codec cod, const uint32_t (&a)[18], uint32_t n,
                                          uint32_t nCells, const TrtC<typename Codec::R> & k,

@@ -35,6 +35,7 @@ void KernelStatistics(KernelData * kd, LatticeDesc * ld, CaseData * cd, int iter
 #include <stdlib.h>
 #include <string.h>
 #include <new>
+#include <vector>
 
 namespace lbm {
 
@@ -276,10 +277,13 @@ __global__ __launch_bounds__(512) void k_list_even_v(typename Codec::S * pdf, Co
 }
 
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, class Adj>
-__global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec cod, Adj adjacency, uint32_t nFluid,
-                                                  uint32_t nCells, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+__global__ __launch_bounds__(512) void k_list_odd(typename Codec::S * pdf, Codec cod, Adj adjacency, OddOrder order,
+                                                  uint32_t nFluid, uint32_t nCells, TrtC<typename Codec::R> k,
+                                                  ForceT<typename Codec::R> ft)
 {
-	const uint32_t n = xcd_grouped_block(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint32_t blk = order.block(blockIdx.x);
+	if (blk == 0xffffffffu) return;             // padding of the band table
+	const uint32_t n = blk * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	uint32_t a[18];
 	adjacency.fetch(n, a);
@@ -494,6 +498,7 @@ struct LaunchCfg {
 	dim3 eblock;
 	size_t elds;
 	dim3 grid, block;
+	OddOrder order;
 	hipStream_t stream;
 	uint32_t nF, nC;
 	bool rle, nt;
@@ -531,9 +536,9 @@ template <class Codec, class Adj>
 static void launch_odd_adj(const LaunchCfg & c, typename Codec::S * pdf, Codec cod, Adj adj, const TrtC<typename Codec::R> & k,
                            const ForceT<typename Codec::R> & ft, bool fl, bool fs)
 {
-	if (fs)      k_list_odd<Codec, true, true><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
-	else if (fl) k_list_odd<Codec, true, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
-	else         k_list_odd<Codec, false, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.nF, c.nC, k, ft);
+	if (fs)      k_list_odd<Codec, true, true><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.order, c.nF, c.nC, k, ft);
+	else if (fl) k_list_odd<Codec, true, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.order, c.nF, c.nC, k, ft);
+	else         k_list_odd<Codec, false, false><<<c.grid, c.block, 0, c.stream>>>(pdf, cod, adj, c.order, c.nF, c.nC, k, ft);
 }
 
 template <class Codec>
@@ -609,14 +614,14 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	if (perLaunch && !common_events(&c, steps)) return false;
 
 	const int bs = c.args.blockSize;
-	const unsigned oddBlocks = (unsigned)((L->nFluid + bs - 1) / bs), groupBlocks = 8 * kXcdGroup;
-	const dim3 grid((oddBlocks + groupBlocks - 1) / groupBlocks * groupBlocks), block((unsigned)bs);
+	const dim3 grid(L->oddGrid), block((unsigned)bs);
+	const OddOrder order = {L->dOddOrder, c.args.oddOrder == 1 ? 1u : 0u};
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
 	const int vec = list_even_vec(L);
 	const StreamShape es = stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
-	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, c.stream, nF, nC,
+	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, order, c.stream, nF, nC,
 	                       c.args.adjRle != 0, c.args.nontemporal != 0};   // default on (-1)
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto even = [&](bool fl) {
@@ -732,6 +737,7 @@ static void list_release(ListData * L)
 	if (!L) return;
 	(void)hipFree(L->dGrid);
 	(void)hipFree(L->dCoords);
+	(void)hipFree(L->dOddOrder);
 	(void)hipFree(L->dAdj);
 	(void)hipFree(L->dRunMask);
 	(void)hipFree(L->dRunBase);
@@ -783,6 +789,53 @@ static bool list_build_runs(ListData * L, hipStream_t s)
 	return ok;
 }
 
+// y of the first node of every odd-step block
+__global__ __launch_bounds__(256) void k_block_y(const uint32_t * __restrict__ coords, uint32_t bs, uint32_t nBlocks,
+                                                 uint32_t * __restrict__ y)
+{
+	const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+	if (b < nBlocks) y[b] = coords[3 * (size_t)b * bs + 1];
+}
+
+// Launch grid and block order of the odd step (OddOrder).  Band order: the
+// device hands launched block b to XCD b % 8; the table gives XCD x the node
+// blocks whose first node lies in y band x (of 8), in ascending order, so
+// the blocks that share lines of the neighbour planes -- x and y neighbours
+// are in the same band -- run in one L2.  Bands hold about equal numbers of
+// blocks; the shorter ones are padded with 0xffffffff (no work).
+static bool list_build_odd_order(ListData * L, hipStream_t s)
+{
+	const uint32_t bs = (uint32_t)L->c.args.blockSize;
+	const uint32_t nb = (uint32_t)((L->nFluid + bs - 1) / bs);
+	if (L->c.args.oddOrder != 2 || nb == 0) {
+		const uint32_t g = L->c.args.oddOrder == 1 ? 8 * kXcdGroup : 1;
+		L->oddGrid = (nb + g - 1) / g * g;
+		return true;
+	}
+	uint32_t * dY = nullptr;
+	if (!dev_alloc((void **)&dY, sizeof(uint32_t) * nb, "block rows")) return false;
+	std::vector<uint32_t> y(nb);
+	k_block_y<<<(nb + 255) / 256, 256, 0, s>>>(L->dCoords, bs, nb, dY);
+	bool ok = hip_ok(hipGetLastError(), "block rows") &&
+	          hip_ok(hipMemcpyAsync(y.data(), dY, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost, s), "block rows") &&
+	          hip_ok(hipStreamSynchronize(s), "block rows");
+	(void)hipFree(dY);
+	if (!ok) return false;
+	std::vector<uint32_t> band[8];
+	const uint64_t Y = (uint64_t)L->dims[1];
+	for (uint32_t b = 0; b < nb; ++b) band[(uint64_t)y[b] * 8 / Y].push_back(b);
+	size_t longest = 0;
+	for (const auto & v : band) longest = v.size() > longest ? v.size() : longest;
+	std::vector<uint32_t> table(8 * longest, 0xffffffffu);
+	for (uint32_t x = 0; x < 8; ++x)
+		for (size_t j = 0; j < band[x].size(); ++j) table[8 * j + x] = band[x][j];
+	L->oddGrid = (uint32_t)table.size();
+	if (!dev_alloc((void **)&L->dOddOrder, sizeof(uint32_t) * table.size(), "odd-step block order")) return false;
+	return hip_ok(hipMemcpyAsync(L->dOddOrder, table.data(), sizeof(uint32_t) * table.size(), hipMemcpyHostToDevice, s),
+	              "odd-step block order") &&
+	       hip_ok(hipStreamSynchronize(s), "odd-step block order");
+}
+
 static bool list_build(ListData * L, const LatticeDesc * ld)
 {
 	const int X = L->dims[0], Y = L->dims[1], Z = L->dims[2];
@@ -829,6 +882,7 @@ static bool list_build(ListData * L, const LatticeDesc * ld)
 			if (!hip_ok(hipMemcpyAsync(&hTotal[1], dSums, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "run count download")) break;
 		}
 		if (L->nFluid > 0 && !list_build_runs(L, s)) break;
+		if (!list_build_odd_order(L, s)) break;
 		L->hGrid = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)N);
 		if (!L->hGrid) { fail("allocating host grid failed: %zu bytes.", sizeof(uint32_t) * (size_t)N); break; }
 		if (!hip_ok(hipMemcpyAsync(L->hGrid, L->dGrid, sizeof(uint32_t) * (size_t)N, hipMemcpyDeviceToHost, s), "grid download")) break;

@@ -49,6 +49,7 @@ struct KernelArgs {
 	int quiet = 0;         // no informational lines at Init
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
+	int oddOrder = 2;      // list: odd-step block order, 0 plain, 1 XCD-grouped, 2 y bands per XCD
 };
 
 // PDF storage of the list kernel.

@@ -194,6 +194,13 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "eager")) a->lazyShadow = 0;
 			else if (!strcmp(v, "lazy")) a->lazyShadow = 1;
 			else return fail("argument -shadow requires eager or lazy.");
+		} else if (listKernel && (is("-odd-order") || is("--odd-order"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -odd-order requires band, group or plain.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "plain")) a->oddOrder = 0;
+			else if (!strcmp(v, "group")) a->oddOrder = 1;
+			else if (!strcmp(v, "band")) a->oddOrder = 2;
+			else return fail("argument -odd-order requires band, group or plain.");
 		} else if (listKernel && (is("-statistics") || is("--statistics"))) {
 			a->statistics = 1;
 		} else if (is("-quiet") || is("--quiet")) {
@@ -213,6 +220,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
+			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			return fail("kernel help requested.");
 		} else {

@@ -237,7 +237,9 @@ def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
     for args in (["-adj", "rle"], ["-adj", "full"], ["-adj", "rle", "-nt", "off"], ["-adj", "full", "-nt", "off"],
                  ["-adj", "rle", "-bs", "128"], ["-adj", "rle", "-bs", "64", "-blk", "5"],
                  ["-adj", "rle", "-even-cap", "off"], ["-adj", "full", "-even-cap", "off", "-bs", "512"],
-                 ["-adj", "rle", "-even-cap", "force"], ["-adj", "full", "-even-cap", "force", "-nt", "off"]):
+                 ["-adj", "rle", "-even-cap", "force"], ["-adj", "full", "-even-cap", "force", "-nt", "off"],
+                 ["-adj", "rle", "-odd-order", "group"], ["-adj", "full", "-odd-order", "plain"],
+                 ["-adj", "rle", "-odd-order", "band", "-bs", "64"]):
         with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
             info = k.list_info()
             assert info.adjRle == (args[1] == "rle")
