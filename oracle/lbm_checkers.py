@@ -235,6 +235,28 @@ class Oracle:
         self.lib.orc_shifted_to_nodes(C.byref(st), store, code, f16_shift, out)
         return out, b
 
+    def list_run_seconds(self, mask, dims, periodic, iterations, omega, xforce, *, magic=1.0 / 12.0,
+                         force_on=False):
+        """Seconds of orc_list_run alone (scalar tree, OpenMP) from all-zero
+        PDFs -- the same work per step as any state; the CPU timing of the
+        restatement."""
+        import time
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        b = self.list_build(mask, dims, periodic)
+        pdf = np.zeros(NDIR * b.n_cells, dtype=np.float64)
+        adj = np.ascontiguousarray(b.adj.reshape(-1))
+        st = _ListT()
+        st.dims = _i3(dims)
+        st.nFluid, st.nCells = b.n_fluid, b.n_cells
+        st.mask = mask.ctypes.data
+        st.grid = b.grid.ctypes.data
+        st.adj = adj.ctypes.data
+        st.pdf = pdf.ctypes.data
+        st.iteration = -1
+        t0 = time.perf_counter()
+        self.lib.orc_list_run(C.byref(st), iterations, omega, magic, xforce, int(force_on), 1, 1)
+        return time.perf_counter() - t0
+
     def aa_run(self, mask, dims, periodic, iterations, omega, xforce, *, magic=1.0 / 12.0,
                force_on=True, init="weights", rho=1.0):
         mask = np.ascontiguousarray(mask, dtype=np.int32)
