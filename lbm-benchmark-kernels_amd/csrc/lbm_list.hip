@@ -19,9 +19,12 @@
 //
 // Device layout: PDFs as the reference (19 planes of nCells doubles; nCells
 // * 8 B is a multiple of 128 B after padding, so every plane starts on a
-// 128-B line).  Adjacency transposed to [18][adjStride] uint32 so that lane
-// n of a wavefront reads entry d of node n next to lane n+1: every
-// adjacency load is one coalesced 256-B access per wavefront.
+// 128-B line; with -prec fp32s / fp16s the planes hold FP32 / FP16
+// deviations f - w instead).  Adjacency transposed to [18][adjStride] uint32
+// so that lane n of a wavefront reads entry d of node n next to lane n+1;
+// the odd step reads a run-compressed copy of it (per 64-node wavefront a
+// run-start mask and base, per run the 18 entries of its first node) and
+// visits its blocks in an XCD-aware order (OddOrder).
 #include "lbm_list.h"
 
 // The program's output functions (the reference's Vtk.o / Kernel.o, or

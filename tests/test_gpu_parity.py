@@ -7,8 +7,8 @@ Bars:
     tree (the GPU uses that tree and no FMA), which for aa-soa is also
     bit-exact against the reference's own output (fixture hashes);
   * list-aa-pv-soa against the reference, whose AVX path uses another
-    association tree for some nodes: relative max-norm <= 1e-12 (the
-    north-star tolerance), as written in REL_TOL.
+    association tree for some nodes: relative max-norm <= 1e-12 after up to
+    1000 steps, as written in REL_TOL.
 """
 import hashlib
 
