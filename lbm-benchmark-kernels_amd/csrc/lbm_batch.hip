@@ -214,6 +214,7 @@ struct Batch {
 	hipGraphExec_t exec = nullptr;
 	int graphSteps = 0;
 	bool graphDirty = true;
+	int tableVec = 1;              // nodes per thread the even chunk list was cut for
 
 	double gpuMs = 0.0, flups = 0.0;
 	int pendingSteps = 0;          // an interval launched and not yet waited for
@@ -296,6 +297,10 @@ static bool rebuild_tables(Batch * b)
 {
 	const int v = batch_vec(b);
 	const uint32_t eSize = (uint32_t)(b->tune.evenBlock * v), oSize = (uint32_t)b->tune.oddBlock;
+	// the captured even launches must take as many nodes per thread as the
+	// chunks are cut for (the set of active domains decides it)
+	if (v != b->tableVec) b->graphDirty = true;
+	b->tableVec = v;
 	b->hEven.clear();
 	b->hOdd.clear();
 	b->hMom.clear();
@@ -420,7 +425,7 @@ static void launch_pair(Batch * b, Codec cod, int v, bool force, bool first, boo
 
 static void launch_steps(Batch * b, int steps)
 {
-	const int v = batch_vec(b);
+	const int v = b->tableVec;
 	const bool force = b->args.forceOn != 0;
 	const float s16 = f16_scale(b);
 	for (int it = 0; it < steps; it += 2) {
@@ -854,7 +859,7 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 		if (b->tablesDirty && !rebuild_tables(b)) return false;
 		(void)hipGetLastError();
 		const float s16 = f16_scale(b);
-		const int v = batch_vec(b);
+		const int v = b->tableVec;
 		if (!hip_ok(hipEventRecord(t0, b->stream), "hipEventRecord")) return false;
 		for (int p = 0; p < pairs; ++p) {
 			if (evenOnly) {

@@ -129,3 +129,25 @@ def test_batch_setnode_guard(lbm):
         lbm.hip_lib().LbmHipClearError()
         kd.contents.SetNode(kd, x, y, z, vals)
         assert lbm.hip_lib().LbmHipLastError().decode() == ""
+
+
+@pytest.mark.parametrize("prec", ["fp32s", "fp16s"])
+def test_batch_nodes_per_thread_change(lbm, prec):
+    # with -pad no a lattice of odd size forces one node per thread in the even
+    # step; when it leaves, the remaining lattices switch to 2 / 4 nodes per
+    # thread and the replayed graph must follow
+    args = ["-prec", prec, "-pad", "no", "-lambda", repr(LAM)]
+    odd = (lbm.Geometry.from_string("porous-3-0.75-9", (21, 17, 13)), 1.1, 2e-5)
+    even = (lbm.Geometry.from_string("porous-2-0.7-5", (20, 18, 16)), 1.3, 1e-5)
+    assert odd[0].n_fluid % 2 == 1 and even[0].n_fluid % 4 == 0
+    with lbm.Batch(args) as b:
+        a = b.add(*odd, max_steps=20)
+        c = b.add(*even)
+        assert b.tuning().vec == 1
+        b.step(20)
+        assert b.status(a).active == 0 and b.tuning().vec == {"fp32s": 2, "fp16s": 4}[prec]
+        b.step(20)
+        b.step(20)
+        runs = [20, 20, 20] if prec == "fp16s" else [60]
+        assert np.array_equal(b.dump(c), _alone(lbm, even[0], even[1], even[2], runs, args))
+        assert np.array_equal(b.dump(a), _alone(lbm, odd[0], odd[1], odd[2], [20], args))
