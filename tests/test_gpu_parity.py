@@ -277,3 +277,16 @@ def test_large_lattice_default_launch(lbm, oracle, name):
     for args in ([], ["-even-cap", "off"]):
         got, _, _ = _run(lbm, name, geo, 10, 1.3, 2e-5, args)
         assert np.array_equal(got, want), args
+
+
+@pytest.mark.parametrize("name", ["list-aa-pv-soa", "aa-soa"])
+def test_index_range_limit(lbm, name):
+    # 19 * cells must fit the 32-bit PDF indices (src/Main.c:399-406); the
+    # check comes before the mask is read, so no mask is needed here
+    ld = lbm.LatticeDesc()
+    ld.Dims = (lbm.C.c_int * 3)(1000, 1000, 300)
+    ld.nCells, ld.nFluid = 300_000_000, 300_000_000
+    geo = lbm.Geometry.__new__(lbm.Geometry)
+    geo.ld, geo._owned_by_c, geo._keep = ld, False, None
+    with pytest.raises(RuntimeError, match="2\\^32"):
+        lbm.Kernel(name, geo)
