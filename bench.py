@@ -280,23 +280,26 @@ def serve_mode(args, lbm):
 
 def aa_mode(args, lbm):
     """aa-soa (full-array AA kernel) on the duct channel: 64^3 x 1000 steps
-    (the small configuration the reference also runs on its CPU) and 256^3."""
+    (the small configuration the reference also runs on its CPU) and 256^3,
+    the latter also with shifted FP32 / FP16 storage."""
     out = {}
-    for edge, steps in ((64, 1000), (256, 200)):
+    for edge, steps, prec in ((64, 1000, "fp64"), (256, 200, "fp64"), (256, 200, "fp32s"), (256, 200, "fp16s")):
         geo = lbm.Geometry.from_string("channel", (edge, edge, edge), (1, 0, 0))
-        with lbm.Kernel("aa-soa", geo, ["-lambda", repr(LAMBDA), "-quiet"] + args.kernel_args.split()) as k:
+        kargs = ["-lambda", repr(LAMBDA), "-quiet", "-prec", prec] + args.kernel_args.split()
+        with lbm.Kernel("aa-soa", geo, kargs) as k:
             k.set_initial_weights()
             k.set_shadow_download(False)
             k.run(lbm.case_data(omega=1.0, xforce=1e-5, iterations=20))
             k.run(lbm.case_data(omega=1.0, xforce=1e-5, iterations=steps))
             t = k.timings()
         nf = geo.n_fluid
-        b = 2 * 19 * 8 + 4          # PDFs plus the cell's info word
-        out[f"{edge}^3"] = {"n_fluid": nf, "steps": steps, "mflups": round(nf * steps / (t.loopMs / 1e3) / 1e6, 1),
-                            "even_ms": round(t.evenMsAvg, 5), "odd_ms": round(t.oddMsAvg, 5),
-                            "even_gbs": round(b * nf / (t.evenMsAvg / 1e3) / 1e9, 1),
-                            "odd_gbs": round(b * nf / (t.oddMsAvg / 1e3) / 1e9, 1)}
-        _log(f"aa-soa {edge}^3: {out[f'{edge}^3']}")
+        b = 2 * 19 * {"fp64": 8, "fp32s": 4, "fp16s": 2}[prec] + 4   # PDFs plus the cell's info word
+        key = f"{edge}^3" + ("" if prec == "fp64" else f" {prec}")
+        out[key] = {"n_fluid": nf, "steps": steps, "mflups": round(nf * steps / (t.loopMs / 1e3) / 1e6, 1),
+                    "even_ms": round(t.evenMsAvg, 5), "odd_ms": round(t.oddMsAvg, 5),
+                    "even_gbs": round(b * nf / (t.evenMsAvg / 1e3) / 1e9, 1),
+                    "odd_gbs": round(b * nf / (t.oddMsAvg / 1e3) / 1e9, 1)}
+        _log(f"aa-soa {key}: {out[key]}")
     return _common("MFLUP/s (D3Q19 TRT aa-soa FP64, 256^3 duct channel)", out["256^3"]["mflups"], "MFLUP/s", 200,
                    "aa-soa, channel (x periodic, walls y/z), omega 1.0, Lambda 3/16, F 1e-5, FP64, f = w at start",
                    {"dtype": "f64", "aa": out})

@@ -30,7 +30,7 @@
  *                     blocks per compute unit; force: at any size; -bs then
  *                     applies to the odd step)
  *   -prec fp64|fp32s|fp16s
- *                     list only: PDF storage.  fp64 (default) is the
+ *                     PDF storage (both kernels).  fp64 (default) is the
  *                     reference's PdfT = double.  fp32s/fp16s store the
  *                     deviation f - w_d (FP16: times 2^k, see -fp16-scale)
  *                     and collide in FP32; no counterpart in the reference,
@@ -39,8 +39,9 @@
  *                     double node views; KernelData.Pdfs[0] then points to
  *                     the storage type.  The body-force increment is added
  *                     in FP32 at the next step's load (after the last step of
- *                     a Kernel() call, at the store).
- *   -fp16-scale <2^k> list only: power-of-two factor of the FP16 deviations
+ *                     a Kernel() call, at the store).  aa-soa's ghost layer
+ *                     starts at f = 0 in every storage type.
+ *   -fp16-scale <2^k> power-of-two factor of the FP16 deviations
  *                     (default 4096)
  *   -statistics       list only: call the program's KernelStatistics during the
  *                     run as the reference's STATISTICS build does; with

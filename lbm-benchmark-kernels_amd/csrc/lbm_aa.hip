@@ -12,17 +12,17 @@
 // put it, so node views are bit-identical to the reference.  Pairs whose
 // target lies outside a non-periodic face have no fix-up in the reference;
 // the value stays in the ghost cell and is read back from there by the same
-// node two steps later.  The ghost layer exists for exactly that case and is
-// zero-initialised (the reference leaves it unset; fresh pages read 0.0).
+// node two steps later.  The ghost layer exists for exactly that case and
+// starts at f = 0 (the reference leaves it unset; fresh pages read 0.0).
 //
 // Device layout: PDF(d, x, y, z) at
-//   d * plane + ((x + 1) * gY + (y + 1)) * zStride + (z + kZOff)
-// gY = Y + 2, plane = (X + 2) * gY * zStride, kZOff = 16 doubles (128 B, the
-// ghost z = -1 sits at 15) and zStride a multiple of 16, so that interior
-// z rows start on a 128-B line and a wavefront's 64 lanes cover four
-// lines.
-#include "lbm_device.h"
-#include "lbm_runtime.h"
+//   d * plane + ((x + 1) * gY + (y + 1)) * zStride + (z + zOff)
+// gY = Y + 2, plane = (X + 2) * gY * zStride, zOff = 128 B of stored values
+// (16 FP64, 32 FP32 or 64 FP16 values; the ghost z = -1 sits at zOff - 1)
+// and zStride a multiple of zOff, so that interior
+// z rows start on a 128-B line and a wavefront's 64 lanes cover whole
+// lines (four, two or one).
+#include "lbm_list.h"      // storage codecs, elem_bytes, kW
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -31,7 +31,6 @@
 
 namespace lbm {
 
-constexpr int kZOff = 16;
 constexpr uint32_t kInfoFluid = 1u << 31;
 constexpr uint32_t kInfoBoundary = 1u << 30;
 
@@ -40,6 +39,7 @@ struct AaGeom {
 	int px, py, pz;
 	int gY;
 	int zStride;
+	int zOff;                // 128 B of stored values
 	uint64_t plane;
 };
 
@@ -50,8 +50,8 @@ struct AaData {
 	int nFluid = 0;
 	int nBounce = 0;
 	uint32_t * dInfo = nullptr;   // [x][y][z], z fastest
-	double * dPdf = nullptr;
-	double * hPdf = nullptr;
+	void * dPdf = nullptr;        // nPdf values of the storage type (-prec)
+	void * hPdf = nullptr;
 	uint32_t * hInfo = nullptr;
 	size_t nPdf = 0;
 };
@@ -60,7 +60,7 @@ static AaData * as_aa(KernelData * kd) { return reinterpret_cast<AaData *>(kd); 
 
 __host__ __device__ __forceinline__ uint64_t aa_slot(const AaGeom & g, int d, int x, int y, int z)
 {
-	return (uint64_t)d * g.plane + ((uint64_t)(x + 1) * g.gY + (uint64_t)(y + 1)) * g.zStride + (uint64_t)(z + kZOff);
+	return (uint64_t)d * g.plane + ((uint64_t)(x + 1) * g.gY + (uint64_t)(y + 1)) * g.zStride + (uint64_t)(z + g.zOff);
 }
 
 // Neighbour (x + dx, y + dy, z + dz) after the reference's periodic rule:
@@ -122,24 +122,25 @@ __device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int 
 // Even step: local load, collision, store of direction d into slot inv(d)
 // of the same cell (src/BenchKernelD3Q19Aa.c:146-290).
 // NT: non-temporal loads and stores (each value is touched once per step).
-template <bool FORCE_LOAD, bool NT>
-__global__ __launch_bounds__(512) void k_aa_even(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
-                                                 TrtC<double> k, ForceT<double> ft)
+template <class Codec, bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
+                                                 AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
+	using R = typename Codec::R;
 	int x, y, z;
 	if (!aa_cell(g, x, y, z)) return;
 	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
 	if (!(w & kInfoFluid)) return;
 	const uint64_t base = aa_slot(g, 0, x, y, z);
-	double f[19];
+	R f[19];
 #pragma unroll
 	for (int d = 0; d < 19; ++d) {
-		f[d] = ld<NT>(&pdf[base + (uint64_t)d * g.plane]);
+		f[d] = cod.template load<NT>(&pdf[base + (uint64_t)d * g.plane]);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	trt_collide(f, k);
 #pragma unroll
-	for (int d = 0; d < 19; ++d) st<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], f[d]);
+	for (int d = 0; d < 19; ++d) cod.template store<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], f[d]);
 }
 
 // Odd step: direction d arrives from the pull neighbour x - c_d, slot inv(d)
@@ -153,10 +154,11 @@ __global__ __launch_bounds__(512) void k_aa_even(double * pdf, const uint32_t * 
 // (its writer would be the ghost cell); that slot is still part of the
 // node view the reference forces after the step, so it gets the increment
 // here too.
-template <bool FORCE_LOAD, bool FORCE_STORE>
-__global__ __launch_bounds__(512) void k_aa_odd(double * pdf, const uint32_t * __restrict__ info, AaGeom g,
-                                                TrtC<double> k, ForceT<double> ft)
+template <class Codec, bool FORCE_LOAD, bool FORCE_STORE>
+__global__ __launch_bounds__(512) void k_aa_odd(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
+                                                AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
+	using R = typename Codec::R;
 	int x, y, z;
 	if (!aa_cell(g, x, y, z)) return;
 	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
@@ -193,19 +195,19 @@ __global__ __launch_bounds__(512) void k_aa_odd(double * pdf, const uint32_t * _
 			dst[d] = aa_slot(g, d, qx, qy, qz);
 		}
 	}
-	double f[19];
+	R f[19];
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
-		f[d] = pdf[src[d]];
+		f[d] = cod.template load<false>(&pdf[src[d]]);
 		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
 	}
 	const uint64_t cSlot = self + (uint64_t)dC * g.plane;
-	f[dC] = pdf[cSlot];
+	f[dC] = cod.template load<false>(&pdf[cSlot]);
 	if (FORCE_LOAD) f[dC] = f[dC] + ft.f[dC];
 	trt_collide(f, k);
 #pragma unroll
 	for (int d = 0; d < 18; ++d) {
-		double v = f[d];
+		R v = f[d];
 		if (FORCE_STORE) {
 			if (w & (1u << d)) {
 				v = v + ft.f[inv_dir(d)];
@@ -216,15 +218,15 @@ __global__ __launch_bounds__(512) void k_aa_odd(double * pdf, const uint32_t * _
 				if (inside) v = v + ft.f[d];
 			}
 		}
-		pdf[dst[d]] = v;
+		cod.template store<false>(&pdf[dst[d]], v);
 	}
-	pdf[cSlot] = FORCE_STORE ? f[dC] + ft.f[dC] : f[dC];
+	cod.template store<false>(&pdf[cSlot], FORCE_STORE ? f[dC] + ft.f[dC] : f[dC]);
 	if (FORCE_STORE && ghostPull) {
 #pragma unroll
 		for (int d = 0; d < 18; ++d) {
 			if (ghostPull & (1u << d)) {
 				const uint64_t own = self + (uint64_t)d * g.plane;
-				pdf[own] = pdf[own] + ft.f[d];
+				cod.template store<false>(&pdf[own], cod.template load<false>(&pdf[own]) + ft.f[d]);
 			}
 		}
 	}
@@ -242,18 +244,30 @@ static bool aa_local_layout(const AaData * A, int x, int y, int z)
 	return true;
 }
 
+static double aa_shadow_get(const AaData * A, uint64_t i, int d)
+{
+	return shadow_value(A->c.args.prec, A->c.args.f16Shift, A->hPdf, (size_t)i, d);
+}
+
+static void aa_shadow_set(AaData * A, uint64_t i, int d, double v)
+{
+	shadow_store(A->c.args.prec, A->c.args.f16Shift, A->hPdf, (size_t)i, d, v);
+}
+
+static size_t aa_bytes(const AaData * A) { return elem_bytes(A->c.args.prec) * A->nPdf; }
+
 static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
 	if (!aa_local_layout(A, x, y, z)) return;
-	for (int d = 0; d < 19; ++d) pdfs[d] = A->hPdf[aa_slot(A->g, d, x, y, z)];
+	for (int d = 0; d < 19; ++d) pdfs[d] = aa_shadow_get(A, aa_slot(A->g, d, x, y, z), d);
 }
 
 static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
 	if (!aa_local_layout(A, x, y, z) || !shadow_writable(&A->c)) return;
-	for (int d = 0; d < 19; ++d) A->hPdf[aa_slot(A->g, d, x, y, z)] = pdfs[d];
+	for (int d = 0; d < 19; ++d) aa_shadow_set(A, aa_slot(A->g, d, x, y, z), d, pdfs[d]);
 	A->c.shadowDirty = 1;
 }
 
@@ -261,15 +275,50 @@ static void aa_bc_get(KernelData * kd, int x, int y, int z, int dir, PdfT * pdf)
 {
 	AaData * A = as_aa(kd);
 	if (!aa_local_layout(A, x, y, z)) return;
-	*pdf = A->hPdf[aa_slot(A->g, dir, x, y, z)];
+	*pdf = aa_shadow_get(A, aa_slot(A->g, dir, x, y, z), dir);
 }
 
 static void aa_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 {
 	AaData * A = as_aa(kd);
 	if (!aa_local_layout(A, x, y, z) || !shadow_writable(&A->c)) return;
-	A->hPdf[aa_slot(A->g, dir, x, y, z)] = pdf;
+	aa_shadow_set(A, aa_slot(A->g, dir, x, y, z), dir, pdf);
 	A->c.shadowDirty = 1;
+}
+
+// One even + odd step pair.  ev: four events around the two launches
+// (per-launch timing) or null.
+struct AaLaunch {
+	dim3 grid, block;        // odd step: one z row segment per block
+	dim3 egrid, eblock;      // even step: blockDim.y rows of y per block
+	size_t elds;
+	hipStream_t stream;
+	const uint32_t * info;
+	AaGeom g;
+	bool nt;
+};
+
+template <class Codec>
+static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, const TrtConsts & k0, const ForceTerms & f0,
+                      bool force, bool fl, bool fs, hipEvent_t * ev)
+{
+	using R = typename Codec::R;
+	const TrtC<R> k = trt_cast<R>(k0);
+	const ForceT<R> ft = force_cast<R>(f0);
+	if (ev) (void)hipEventRecord(ev[0], L.stream);
+	if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	if (ev) {
+		(void)hipEventRecord(ev[1], L.stream);
+		(void)hipEventRecord(ev[2], L.stream);
+	}
+	// the odd step loads the force increment of the even step whenever the force is on
+	if (fs)          k_aa_odd<Codec, true, true><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else if (force)  k_aa_odd<Codec, true, false><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else             k_aa_odd<Codec, false, false><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	if (ev) (void)hipEventRecord(ev[3], L.stream);
 }
 
 // --------------------------------------------------------------- Kernel()
@@ -277,7 +326,7 @@ static bool aa_download(KernelData * kd)
 {
 	AaData * A = as_aa(kd);
 	if (A->c.shadowDirty) return fail("host lattice modified through SetNode since the last Kernel() call; not overwriting it.");
-	if (!hip_ok(hipMemcpyAsync(A->hPdf, A->dPdf, sizeof(double) * A->nPdf, hipMemcpyDeviceToHost, A->c.stream), "download of the device lattice")) return false;
+	if (!hip_ok(hipMemcpyAsync(A->hPdf, A->dPdf, aa_bytes(A), hipMemcpyDeviceToHost, A->c.stream), "download of the device lattice")) return false;
 	if (!hip_ok(hipStreamSynchronize(A->c.stream), "hipStreamSynchronize")) return false;
 	A->c.shadowStale = 0;
 	return true;
@@ -299,7 +348,7 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 
 	if (c.shadowDirty) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, sizeof(double) * A->nPdf, hipMemcpyHostToDevice, c.stream), "upload of the host lattice")) return;
+		if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, aa_bytes(A), hipMemcpyHostToDevice, c.stream), "upload of the host lattice")) return;
 		if (!hip_ok(hipStreamSynchronize(c.stream), "hipStreamSynchronize")) return;
 		c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
 		c.shadowDirty = 0;
@@ -307,8 +356,8 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 		c.t.uploadMs = 0.0;
 	}
 
-	const TrtC<double> k = trt_cast<double>(trt_consts(cd->Omega, c.args.magic));
-	const ForceT<double> ft = force_cast<double>(force_terms(cd->XForce));
+	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
+	const ForceTerms ft = force_terms(cd->XForce);
 	const bool force = c.args.forceOn != 0;
 	const bool perLaunch = c.launchEvents != 0;
 	if (perLaunch && !common_events(&c, steps)) return;
@@ -323,25 +372,23 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const dim3 eblock((unsigned)bs, rows), egrid(grid.x, ((unsigned)g.Y + rows - 1) / rows, grid.z);
 	// non-temporal accesses measured slower here than in the list kernel: off unless asked for
 	const bool nt = c.args.nontemporal == 1;
+	const AaLaunch L = {grid, block, egrid, eblock, es.lds, c.stream, A->dInfo, g, nt};
+	const float f16s = (float)(1 << c.args.f16Shift);
+	auto pair = [&](bool on, bool fl, bool fs, int it) {
+		switch (c.args.prec) {
+			case PREC_FP32S: aa_launch(L, (float *)A->dPdf, CodecF32{}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr); break;
+			case PREC_FP16S:
+				aa_launch(L, (uint16_t *)A->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr);
+				break;
+			default: aa_launch(L, (double *)A->dPdf, CodecF64{}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr);
+		}
+	};
 
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
 	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
-		if (perLaunch) (void)hipEventRecord(c.ev[2 * it], c.stream);
-		const bool fl = force && it > 0;
-		if (fl && nt)  k_aa_even<true, true><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else if (fl)   k_aa_even<true, false><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else if (nt)   k_aa_even<false, true><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else           k_aa_even<false, false><<<egrid, eblock, es.lds, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		if (perLaunch) {
-			(void)hipEventRecord(c.ev[2 * it + 1], c.stream);
-			(void)hipEventRecord(c.ev[2 * it + 2], c.stream);
-		}
-		if (force && last) k_aa_odd<true, true><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else if (force)    k_aa_odd<true, false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		else               k_aa_odd<false, false><<<grid, block, 0, c.stream>>>(A->dPdf, A->dInfo, g, k, ft);
-		if (perLaunch) (void)hipEventRecord(c.ev[2 * it + 3], c.stream);
+		pair(force, force && it > 0, force && last, it);
 	}
 	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
 	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
@@ -395,7 +442,8 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	g.px = ld->PeriodicX; g.py = ld->PeriodicY; g.pz = ld->PeriodicZ;
 	if (g.X <= 0 || g.Y <= 0 || g.Z <= 0) { fail("empty dimensions."); aa_release(A); return; }
 	g.gY = g.Y + 2;
-	g.zStride = ((kZOff + g.Z + 1 + 15) / 16) * 16;
+	g.zOff = (int)(128 / elem_bytes(A->c.args.prec));
+	g.zStride = ((g.zOff + g.Z + 1 + g.zOff - 1) / g.zOff) * g.zOff;
 	g.plane = (uint64_t)(g.X + 2) * g.gY * g.zStride;
 	A->nPdf = (size_t)19 * g.plane;
 	A->nFluid = ld->nFluid;
@@ -405,7 +453,7 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	if (!A->c.args.quiet) {
 		printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);
 		printf("# allocating data for %llu LB nodes with padding (%llu bytes = %f MiB for the single lattice)\n",
-		       (unsigned long long)(g.plane), (unsigned long long)(A->nPdf * 8), A->nPdf * 8.0 / 1024.0 / 1024.0);
+		       (unsigned long long)(g.plane), (unsigned long long)aa_bytes(A), aa_bytes(A) / 1024.0 / 1024.0);
 	}
 
 	int * dMask = nullptr;
@@ -417,17 +465,24 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		if (!hip_ok(hipMalloc((void **)&dMask, sizeof(int) * nTotal), "allocating geometry mask")) break;
 		if (!hip_ok(hipMalloc((void **)&dCount, sizeof(uint32_t)), "allocating counter")) break;
 		if (!hip_ok(hipMalloc((void **)&A->dInfo, sizeof(uint32_t) * nTotal), "allocating cell info")) break;
-		if (!hip_ok(hipMalloc((void **)&A->dPdf, sizeof(double) * A->nPdf), "allocating device lattice")) break;
+		if (!hip_ok(hipMalloc((void **)&A->dPdf, aa_bytes(A)), "allocating device lattice")) break;
 		if (!hip_ok(hipMemcpyAsync(dMask, ld->Lattice, sizeof(int) * nTotal, hipMemcpyHostToDevice, s), "mask upload")) break;
 		if (!hip_ok(hipMemsetAsync(dCount, 0, sizeof(uint32_t), s), "hipMemset")) break;
-		if (!hip_ok(hipMemsetAsync(A->dPdf, 0, sizeof(double) * A->nPdf, s), "hipMemset")) break;
 		(void)hipGetLastError();
 		k_aa_info<<<(unsigned)((nTotal + 255) / 256), 256, 0, s>>>(dMask, g, A->dInfo, dCount);
 		if (!hip_ok(hipGetLastError(), "cell info kernel")) break;
 		if (!hip_ok(hipMemcpyAsync(&hCount, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count download")) break;
-		A->hPdf = (double *)host_alloc(sizeof(double) * A->nPdf);
-		if (!A->hPdf) { fail("allocating host lattice failed: %zu bytes.", sizeof(double) * A->nPdf); break; }
-		memset(A->hPdf, 0, sizeof(double) * A->nPdf);
+		A->hPdf = host_alloc(aa_bytes(A));
+		if (!A->hPdf) { fail("allocating host lattice failed: %zu bytes.", aa_bytes(A)); break; }
+		// every slot starts as f = 0, ghost cells included; shifted storage
+		// holds that as the code of -w_d
+		if (A->c.args.prec == PREC_FP64) {
+			memset(A->hPdf, 0, aa_bytes(A));
+		} else {
+			for (int d = 0; d < 19; ++d)
+				for (uint64_t i = 0; i < g.plane; ++i) aa_shadow_set(A, (uint64_t)d * g.plane + i, d, 0.0);
+		}
+		if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, aa_bytes(A), hipMemcpyHostToDevice, s), "lattice upload")) break;
 		if (!hip_ok(hipStreamSynchronize(s), "init")) break;
 		ok = true;
 	} while (0);
@@ -438,11 +493,11 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	if (!A->c.args.quiet) printf("# bounce back pdfs: %d (resolved in the odd step, no index lists)\n", A->nBounce);
 
 	KernelData * kd = &A->kd;
-	kd->Pdfs[0] = A->hPdf;
-	kd->PdfsActive = A->hPdf;
+	kd->Pdfs[0] = (PdfT *)A->hPdf;     // storage type given by -prec (double by default)
+	kd->PdfsActive = (PdfT *)A->hPdf;
 	for (int i = 0; i < 3; ++i) { kd->Dims[i] = ld->Dims[i]; kd->GlobalDims[i] = ld->Dims[i] + 2; kd->Offsets[i] = 1; }
 	kd->nBounceBackPdfs = A->nBounce;
-	kd->LoopBalance = 2.0 * 19 * 8 + 4.0;
+	kd->LoopBalance = 2.0 * 19 * (double)elem_bytes(A->c.args.prec) + 4.0;
 	kd->GetNode = aa_get_node;
 	kd->SetNode = aa_set_node;
 	kd->BoundaryConditionsGetPdf = aa_bc_get;

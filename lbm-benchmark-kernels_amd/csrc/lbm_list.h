@@ -44,6 +44,11 @@ bool list_upload(ListData * L);
 bool list_download(ListData * L);
 // Bytes of one stored value for a -prec setting.
 size_t elem_bytes(int prec);
+
+// Host copy element <-> node-view value (f) for a -prec setting: FP64
+// values as stored, shifted storage as w_d + stored (FP16: scaled by 2^-k).
+double shadow_value(int prec, int f16Shift, const void * h, size_t i, int d);
+void shadow_store(int prec, int f16Shift, void * h, size_t i, int d, double v);
 // Nodes per thread of the even step (1, or 2 / 4 for FP32 / FP16 storage).
 int list_even_vec(const ListData * L);
 // Lattice weights w_d in direction order.

@@ -372,24 +372,32 @@ const double kW[19] = {
 
 size_t elem_bytes(int prec) { return prec == PREC_FP64 ? 8 : prec == PREC_FP32S ? 4 : 2; }
 
+double shadow_value(int prec, int f16Shift, const void * h, size_t i, int d)
+{
+	switch (prec) {
+		case PREC_FP32S: return kW[d] + (double)((const float *)h)[i];
+		case PREC_FP16S: return kW[d] + (double)(f16_to_f32(((const uint16_t *)h)[i]) / (float)(1 << f16Shift));
+		default: return ((const double *)h)[i];
+	}
+}
+
+void shadow_store(int prec, int f16Shift, void * h, size_t i, int d, double v)
+{
+	switch (prec) {
+		case PREC_FP32S: ((float *)h)[i] = (float)(v - kW[d]); break;
+		case PREC_FP16S: ((uint16_t *)h)[i] = f32_to_f16((float)(v - kW[d]) * (float)(1 << f16Shift)); break;
+		default: ((double *)h)[i] = v;
+	}
+}
+
 static double shadow_get(const ListData * L, size_t i, int d)
 {
-	switch (L->c.args.prec) {
-		case PREC_FP32S: return kW[d] + (double)((const float *)L->hPdf)[i];
-		case PREC_FP16S: return kW[d] + (double)(f16_to_f32(((const uint16_t *)L->hPdf)[i]) / (float)(1 << L->c.args.f16Shift));
-		default: return ((const double *)L->hPdf)[i];
-	}
+	return shadow_value(L->c.args.prec, L->c.args.f16Shift, L->hPdf, i, d);
 }
 
 static void shadow_set(ListData * L, size_t i, int d, double v)
 {
-	switch (L->c.args.prec) {
-		case PREC_FP32S: ((float *)L->hPdf)[i] = (float)(v - kW[d]); break;
-		case PREC_FP16S:
-			((uint16_t *)L->hPdf)[i] = f32_to_f16((float)(v - kW[d]) * (float)(1 << L->c.args.f16Shift));
-			break;
-		default: ((double *)L->hPdf)[i] = v;
-	}
+	shadow_store(L->c.args.prec, L->c.args.f16Shift, L->hPdf, i, d, v);
 }
 
 // Node id of a fluid cell, or false (with the error recorded) for a cell

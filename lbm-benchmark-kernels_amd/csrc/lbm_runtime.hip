@@ -168,14 +168,14 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "on")) a->nontemporal = 1;
 			else if (!strcmp(v, "off")) a->nontemporal = 0;
 			else return fail("argument -nt requires on or off.");
-		} else if (listKernel && (is("-prec") || is("--prec"))) {
+		} else if (is("-prec") || is("--prec")) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -prec requires fp64, fp32s or fp16s.");
 			const char * v = p->KernelArgs[++i];
 			if (!strcmp(v, "fp64")) a->prec = PREC_FP64;
 			else if (!strcmp(v, "fp32s")) a->prec = PREC_FP32S;
 			else if (!strcmp(v, "fp16s")) a->prec = PREC_FP16S;
 			else return fail("argument -prec requires fp64, fp32s or fp16s.");
-		} else if (listKernel && (is("-fp16-scale") || is("--fp16-scale"))) {
+		} else if (is("-fp16-scale") || is("--fp16-scale")) {
 			int v;
 			if (!arg_int(p, i, &v, "-fp16-scale", 1, 1 << 24)) return false;
 			if (v & (v - 1)) return fail("argument -fp16-scale must be a power of two.");
@@ -219,7 +219,8 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			if (listKernel) printf("  [-adj rle|full] [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
+			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>]\n");
+			if (listKernel) printf("  [-adj rle|full] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
 			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			return fail("kernel help requested.");

@@ -106,6 +106,11 @@ class Oracle:
                                            C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_shifted_from_nodes.argtypes = [C.POINTER(_ListT), _f32p, C.c_int, C.c_int, _f64p]
         L.orc_shifted_to_nodes.argtypes = [C.POINTER(_ListT), _f32p, C.c_int, C.c_int, _f64p]
+        L.orc_aa_run_shifted.argtypes = [C.POINTER(_AaT), _f32p, C.c_int, C.c_int, C.c_int,
+                                         C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_aa_run_shifted.restype = C.c_int
+        L.orc_aa_shifted_from_nodes.argtypes = [C.POINTER(_AaT), _f32p, C.c_int, C.c_int, _f64p]
+        L.orc_aa_shifted_to_nodes.argtypes = [C.POINTER(_AaT), _f32p, C.c_int, C.c_int, _f64p]
         self.CX = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CX"))
         self.CY = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CY"))
         self.CZ = np.ctypeslib.as_array((C.c_int * NDIR).in_dll(L, "ORC_CZ"))
@@ -269,6 +274,33 @@ class Oracle:
             out = np.zeros((int(np.count_nonzero(mask)), NDIR))
             self.lib.orc_aa_dump(C.byref(st), out)
             return out, st.nBB
+        finally:
+            self.lib.orc_aa_free(C.byref(st))
+
+    def aa_run_shifted(self, mask, dims, periodic, runs, omega, xforce, prec, *, f16_shift=12,
+                       magic=1.0 / 12.0, force_on=True, nodes=None):
+        """Shifted-storage aa-soa (prec "fp32s" or "fp16s"), runs and initial
+        node views as in list_run_shifted.  Returns the node views after the
+        last run (nFluid x 19, mask order)."""
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        nf = int(np.count_nonzero(mask))
+        code = self.PREC[prec]
+        st = _AaT()
+        if self.lib.orc_aa_init(C.byref(st), mask, _i3(dims), _i3(periodic)) != 0:
+            raise MemoryError("orc_aa_init")
+        try:
+            store = np.zeros(NDIR * int(np.prod(np.asarray(dims) + 2)), dtype=np.float32)
+            if nodes is None:
+                nodes = np.tile(self.W, (nf, 1))
+            self.lib.orc_aa_shifted_from_nodes(C.byref(st), store, code, f16_shift,
+                                               np.ascontiguousarray(nodes, dtype=np.float64))
+            for steps in ([runs] if isinstance(runs, int) else runs):
+                if self.lib.orc_aa_run_shifted(C.byref(st), store, code, f16_shift, int(steps), omega,
+                                               magic, xforce, int(force_on)) != 0:
+                    raise MemoryError("orc_aa_run_shifted")
+            out = np.zeros((nf, NDIR))
+            self.lib.orc_aa_shifted_to_nodes(C.byref(st), store, code, f16_shift, out)
+            return out
         finally:
             self.lib.orc_aa_free(C.byref(st))
 

@@ -834,3 +834,115 @@ void orc_shifted_to_nodes(const orc_list_t * L, const float * st, int prec, int 
 		++i;
 	}
 }
+
+/* aa-soa with shifted storage: the steps of orc_aa_run (even step, fix-up
+ * copies, odd step, fix-up copies) over float slots laid out like A->pdf,
+ * with the collision and the folded force increments of
+ * orc_list_run_shifted.  The fix-ups move stored codes unchanged.  The last
+ * odd step of a run keeps its collision outputs in float until the fix-ups
+ * have put them in place, so that the node-view force increment is added
+ * before the one rounding to the storage code; a node-view slot that the
+ * step does not write (its writer would sit in the ghost layer) gets the
+ * increment by decode, add, encode.  Slots outside node views that the last
+ * step writes are stored without increment. */
+int orc_aa_run_shifted(const orc_aa_t * A, float * st, int prec, int f16Shift, int iterations,
+                       double omega, double magic, double F, int forceOn)
+{
+	const trt_f k = trt_f_consts(omega, magic);
+	const codec_f cd = make_codec(prec, f16Shift);
+	const int X = A->dims[0], Y = A->dims[1], Z = A->dims[2];
+	const size_t n = (size_t)ORC_NDIR * A->gdims[0] * A->gdims[1] * A->gdims[2];
+	float ft[ORC_NDIR];
+	for (int d = 0; d < ORC_NDIR; ++d) ft[d] = (float)orc_force_term(d, F);
+	float * raw = NULL;
+	unsigned char * wr = NULL;
+
+	for (int it = 0; it < iterations; it += 2) {
+		const int loadForce = forceOn && it > 0, fold = forceOn && it + 2 >= iterations;
+		#pragma omp parallel for schedule(static)
+		for (int x = 0; x < X; ++x) for (int y = 0; y < Y; ++y) for (int z = 0; z < Z; ++z) {
+			if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
+			float p[ORC_NDIR], q[ORC_NDIR];
+			for (int d = 0; d < ORC_NDIR; ++d) {
+				p[d] = cload(&cd, st[p5(A, x + 1, y + 1, z + 1, d)]);
+				if (loadForce) p[d] = p[d] + ft[d];
+			}
+			trt_collide_f(p, q, &k);
+			for (int d = 0; d < ORC_NDIR; ++d) st[p5(A, x + 1, y + 1, z + 1, ORC_INV[d])] = cstore(&cd, q[d]);
+		}
+		for (int i = 0; i < A->nBB; ++i) st[A->bbSrc[i]] = st[A->bbDst[i]];
+
+		if (fold && !raw) {
+			raw = (float *)calloc(n, sizeof(float));
+			wr = (unsigned char *)calloc(n, 1);
+			if (!raw || !wr) { free(raw); free(wr); return -1; }
+		}
+		#pragma omp parallel for schedule(static)
+		for (int x = 0; x < X; ++x) for (int y = 0; y < Y; ++y) for (int z = 0; z < Z; ++z) {
+			if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
+			float p[ORC_NDIR], q[ORC_NDIR];
+			for (int d = 0; d < ORC_NDIR; ++d) {
+				p[d] = cload(&cd, st[p5(A, x + 1 - ORC_CX[d], y + 1 - ORC_CY[d], z + 1 - ORC_CZ[d], ORC_INV[d])]);
+				if (forceOn) p[d] = p[d] + ft[d];
+			}
+			trt_collide_f(p, q, &k);
+			for (int d = 0; d < ORC_NDIR; ++d) {
+				const size_t s = p5(A, x + 1 + ORC_CX[d], y + 1 + ORC_CY[d], z + 1 + ORC_CZ[d], d);
+				if (fold) { raw[s] = q[d]; wr[s] = 1; }
+				else      st[s] = cstore(&cd, q[d]);
+			}
+		}
+		if (!fold) {
+			for (int i = 0; i < A->nBB; ++i) st[A->bbDst[i]] = st[A->bbSrc[i]];
+			continue;
+		}
+		for (int i = 0; i < A->nBB; ++i) { raw[A->bbDst[i]] = raw[A->bbSrc[i]]; wr[A->bbDst[i]] = wr[A->bbSrc[i]]; }
+		for (int x = 0; x < X; ++x) for (int y = 0; y < Y; ++y) for (int z = 0; z < Z; ++z) {
+			if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
+			for (int d = 0; d < ORC_NDIR; ++d) {
+				const size_t s = p5(A, x + 1, y + 1, z + 1, d);
+				st[s] = cstore(&cd, (wr[s] ? raw[s] : cload(&cd, st[s])) + ft[d]);
+				wr[s] = 2;
+			}
+		}
+		for (size_t s = 0; s < n; ++s) {
+			if (wr[s] == 1) st[s] = cstore(&cd, raw[s]);
+			wr[s] = 0;
+		}
+	}
+	free(raw);
+	free(wr);
+	return 0;
+}
+
+/* Shifted storage of the aa layout: every slot at f = 0 (ghost cells
+ * included), then the node views of the fluid cells (mask order) in their
+ * local slots. */
+void orc_aa_shifted_from_nodes(const orc_aa_t * A, float * st, int prec, int f16Shift, const double * nodes)
+{
+	const codec_f cd = make_codec(prec, f16Shift);
+	const size_t plane = (size_t)A->gdims[0] * A->gdims[1] * A->gdims[2];
+	for (int d = 0; d < ORC_NDIR; ++d) {
+		const float zero = cstore(&cd, (float)(0.0 - ORC_W[d]));
+		for (size_t i = 0; i < plane; ++i) st[(size_t)d * plane + i] = zero;
+	}
+	size_t i = 0;
+	for (int z = 0; z < A->dims[2]; ++z) for (int y = 0; y < A->dims[1]; ++y) for (int x = 0; x < A->dims[0]; ++x) {
+		if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
+		for (int d = 0; d < ORC_NDIR; ++d)
+			st[p5(A, x + 1, y + 1, z + 1, d)] = cstore(&cd, (float)(nodes[i * ORC_NDIR + d] - ORC_W[d]));
+		++i;
+	}
+}
+
+void orc_aa_shifted_to_nodes(const orc_aa_t * A, const float * st, int prec, int f16Shift, double * nodes)
+{
+	const codec_f cd = make_codec(prec, f16Shift);
+	size_t i = 0;
+	for (int z = 0; z < A->dims[2]; ++z) for (int y = 0; y < A->dims[1]; ++y) for (int x = 0; x < A->dims[0]; ++x) {
+		if (A->mask[lidx(A->dims, x, y, z)] == 0) continue;
+		for (int d = 0; d < ORC_NDIR; ++d)
+			nodes[i * ORC_NDIR + d] = ORC_W[d] + (double)cload(&cd, st[p5(A, x + 1, y + 1, z + 1, d)]);
+		++i;
+	}
+}

@@ -33,6 +33,13 @@
  *                        the reference's vector split), mass conservation,
  *                        analytic Poiseuille profile
  *   orc_verify           the reference's own -verify deviation value
+ *   orc_list_run_shifted,
+ *   orc_aa_run_shifted   FP16 rounding against numpy's binary16, rest state
+ *                        as a fixed point, tracking of the FP64 runs within
+ *                        stated tolerances, run-split invariance, and the
+ *                        two shifted runs bit-identical to each other where
+ *                        their boundary handling coincides (no open faces)
+ *                        (test_oracle_shifted.py)
  */
 #ifndef LBM_ORACLE_H
 #define LBM_ORACLE_H
@@ -177,6 +184,13 @@ void orc_list_run_shifted(const orc_list_t * L, float * store, int prec, int f16
 void orc_shifted_from_nodes(const orc_list_t * L, float * store, int prec, int f16Shift, const double * nodes);
 void orc_shifted_to_nodes(const orc_list_t * L, const float * store, int prec, int f16Shift, double * nodes);
 void orc_aa_dump(const orc_aa_t * A, double * out);
+/* aa-soa over shifted storage laid out like orc_aa_t.pdf (float per slot,
+ * every slot -- ghost cells included -- starting at f = 0).  Returns -1 when
+ * the scratch copy of the last odd step cannot be allocated. */
+int  orc_aa_run_shifted(const orc_aa_t * A, float * store, int prec, int f16Shift, int iterations,
+                        double omega, double magic, double xForce, int forceOn);
+void orc_aa_shifted_from_nodes(const orc_aa_t * A, float * store, int prec, int f16Shift, const double * nodes);
+void orc_aa_shifted_to_nodes(const orc_aa_t * A, const float * store, int prec, int f16Shift, double * nodes);
 
 #ifdef __cplusplus
 }

@@ -139,11 +139,22 @@ def test_kernel_argument_errors(lbm, args, msg):
         lbm.Kernel("list-aa-pv-soa", geo, args)
 
 
-@pytest.mark.parametrize("args", [["-pad", "auto"], ["-prec", "fp32s"], ["-fp16-scale", "1024"], ["-adj", "full"],
-                                  ["-vec", "1"]])
+@pytest.mark.parametrize("args", [["-pad", "auto"], ["-adj", "full"], ["-vec", "1"], ["-shadow", "lazy"],
+                                  ["-odd-order", "band"], ["-statistics"]])
 def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):
+        lbm.Kernel("aa-soa", geo, args)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-prec", "fp8"], "fp64, fp32s or fp16s"),
+    (["-fp16-scale", "3000"], "power of two"),
+    (["-even-cap", "maybe"], "on, off or force"),
+])
+def test_aa_argument_errors(lbm, args, msg):
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with pytest.raises(RuntimeError, match=msg):
         lbm.Kernel("aa-soa", geo, args)
 
 

@@ -169,3 +169,63 @@ def test_porous256_shifted_accuracy(lbm):
         assert vel < ACC_TOL[prec][0]
         assert perm < ACC_TOL[prec][1]
         assert abs(mass - geo.n_fluid) / geo.n_fluid < 1e-6
+
+
+# ---- aa-soa with shifted storage ---------------------------------------
+AA_CASES = ["fluid6", "blocks2_12", "box8", "channel10x9x7", "porous16"]
+
+
+def _aa_gpu(lbm, geo, runs, omega, F, args, nodes=None):
+    with lbm.Kernel("aa-soa", geo, list(args)) as k:
+        if nodes is None:
+            k.set_initial_weights()
+        else:
+            k.load(nodes)
+        for steps in runs:
+            k.run(lbm.case_data(omega=omega, xforce=F, iterations=steps))
+        return k.dump()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("case", AA_CASES)
+def test_aa_shifted_bitexact(lbm, oracle, case, prec):
+    """Against the oracle's shifted aa-soa restatement (fix-up copies around
+    full-array steps; open faces through the ghost layer) -- node views
+    bit-exact, across launch shapes and split Kernel() calls."""
+    g = golden(f"tiny_{case}.npz")
+    dims, per = tuple(int(v) for v in g["dims"]), tuple(int(v) for v in g["periodic"])
+    mask = g["mask"].astype(np.int32)
+    geo = lbm.Geometry.from_mask(mask, dims, per)
+    omega, F = float(g["omega"]), float(g["xforce"])
+    base = ["-prec", prec, "-lambda", repr(LAM)]
+    want = oracle.aa_run_shifted(mask, dims, per, [60, 40], omega, F, prec, magic=LAM)
+    for extra in ([], ["-nt", "on"], ["-even-cap", "force"], ["-even-cap", "off", "-bs", "64"]):
+        print(f"aa case {case} -prec {prec} {extra}", flush=True)
+        got = _aa_gpu(lbm, geo, [60, 40], omega, F, base + extra)
+        assert np.array_equal(got, want), (extra, np.abs(got - want).max())
+    want_off = oracle.aa_run_shifted(mask, dims, per, 30, omega, F, prec, magic=LAM, force_on=False)
+    got = _aa_gpu(lbm, geo, [30], omega, F, base + ["-force", "off"])
+    assert np.array_equal(got, want_off)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_aa_shifted_long_rows_and_nodes(lbm, oracle, prec):
+    """z rows longer than one block (300 cells), x periodic, y/z solid
+    faces; random initial node views through SetNode; FP16 scale option."""
+    geo = lbm.Geometry.from_string("porous-3-0.75-9", (24, 20, 300))
+    m = geo.mask.reshape(-1)
+    rng = np.random.default_rng(11)
+    nodes = np.tile(oracle.W, (geo.n_fluid, 1)) + 1e-3 * rng.standard_normal((geo.n_fluid, 19))
+    print(f"aa porous 24x20x300 -prec {prec}", flush=True)
+    want = oracle.aa_run_shifted(m, geo.dims, geo.periodic, [40, 20], 1.25, 3e-5, prec, magic=LAM, nodes=nodes)
+    got = _aa_gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)], nodes=nodes)
+    assert np.array_equal(got, want)
+    # the list kernel gives the same bits here (no open faces)
+    got_list, _ = _gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)], nodes=nodes)
+    assert np.array_equal(got_list, want)
+    if prec == "fp16s":
+        want = oracle.aa_run_shifted(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, prec, magic=LAM, nodes=nodes,
+                                     f16_shift=10)
+        got = _aa_gpu(lbm, geo, [30], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM), "-fp16-scale", "1024"],
+                      nodes=nodes)
+        assert np.array_equal(got, want)
