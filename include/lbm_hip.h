@@ -56,10 +56,11 @@
  *                     blocks of one eighth of the y range; group: 8
  *                     consecutive blocks per XCD in turn; plain: launch order
  *   -quiet            no informational lines at Init
- *   -vec auto|1       list only, fp32s/fp16s: the even step takes 2 (FP32)
- *                     or 4 (FP16) nodes per thread with 8-byte accesses when
- *                     the padded node count allows (auto, default), or one
- *                     node per thread (1)
+ *   -vec auto|1       fp32s/fp16s: the even step takes 2 (FP32) or 4 (FP16)
+ *                     nodes (aa-soa: consecutive z cells) per thread with
+ *                     8-byte accesses (auto, default; the list kernel only
+ *                     when the padded node count allows), or one per
+ *                     thread (1)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H

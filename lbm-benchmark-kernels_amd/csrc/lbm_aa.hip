@@ -143,6 +143,60 @@ __global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec 
 	for (int d = 0; d < 19; ++d) cod.template store<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], f[d]);
 }
 
+// Even step over V consecutive cells z0 .. z0+V-1 of a z row, for the
+// narrow storage types: one 8-byte access per direction carries V values
+// (V = 2 for FP32, 4 for FP16).  zOff, zStride and plane are multiples of V,
+// so the accesses are aligned.  Cells of the group that are solid or past
+// the row end (ghost z = Z, padding) get their values written back
+// unchanged: the slot that takes direction d of a fluid cell, inv(d), takes
+// the cell's own value inv(d) (a decode/encode round trip is exact).  Each
+// fluid cell's arithmetic is that of k_aa_even.
+template <class Codec, int V, bool FORCE_LOAD, bool NT>
+__global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
+                                                   AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+{
+	using R = typename Codec::R;
+	using VT = VecS<typename Codec::S, V>;
+	const int z0 = (int)(blockIdx.x * blockDim.x + threadIdx.x) * V;
+	const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+	const int x = (int)blockIdx.z;
+	if (z0 >= g.Z || y >= g.Y) return;
+	const size_t i0 = ((size_t)x * g.Y + y) * g.Z + z0;
+	bool fluid[V];
+	bool any = false;
+#pragma unroll
+	for (int j = 0; j < V; ++j) {
+		fluid[j] = z0 + j < g.Z && (info[i0 + j] & kInfoFluid);
+		any = any || fluid[j];
+	}
+	if (!any) return;
+	const uint64_t base = aa_slot(g, 0, x, y, z0);
+	R f[V][19];
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		const VT v = ld<NT>(reinterpret_cast<const VT *>(&pdf[base + (uint64_t)d * g.plane]));
+#pragma unroll
+		for (int j = 0; j < V; ++j) f[j][d] = cod.dec(v[j]);
+	}
+#pragma unroll
+	for (int j = 0; j < V; ++j) {
+		if (fluid[j]) {
+			if (FORCE_LOAD) {
+#pragma unroll
+				for (int d = 0; d < 19; ++d) f[j][d] = f[j][d] + ft.f[d];
+			}
+			trt_collide(f[j], k);
+		}
+	}
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		VT v;
+#pragma unroll
+		for (int j = 0; j < V; ++j) v[j] = cod.enc(fluid[j] ? f[j][d] : f[j][inv_dir(d)]);
+		st<NT>(reinterpret_cast<VT *>(&pdf[base + (uint64_t)inv_dir(d) * g.plane]), v);
+	}
+}
+
 // Odd step: direction d arrives from the pull neighbour x - c_d, slot inv(d)
 // (src/BenchKernelD3Q19Aa.c:392) -- or from the cell's own slot d when that
 // neighbour is solid -- and leaves to the push neighbour x + c_d, slot d
@@ -292,6 +346,7 @@ struct AaLaunch {
 	dim3 grid, block;        // odd step: one z row segment per block
 	dim3 egrid, eblock;      // even step: blockDim.y rows of y per block
 	size_t elds;
+	int vec;                 // even step: cells per thread (1, or VMAX of the codec)
 	hipStream_t stream;
 	const uint32_t * info;
 	AaGeom g;
@@ -305,11 +360,24 @@ static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, co
 	using R = typename Codec::R;
 	const TrtC<R> k = trt_cast<R>(k0);
 	const ForceT<R> ft = force_cast<R>(f0);
+	constexpr int V = Codec::VMAX;
 	if (ev) (void)hipEventRecord(ev[0], L.stream);
-	if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	bool vecDone = false;
+	if constexpr (V > 1) {
+		if (L.vec == V) {
+			vecDone = true;
+			if (fl && L.nt)  k_aa_even_v<Codec, V, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			else if (fl)     k_aa_even_v<Codec, V, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			else if (L.nt)   k_aa_even_v<Codec, V, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			else             k_aa_even_v<Codec, V, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+		}
+	}
+	if (!vecDone) {
+		if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+		else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+		else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+		else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	}
 	if (ev) {
 		(void)hipEventRecord(ev[1], L.stream);
 		(void)hipEventRecord(ev[2], L.stream);
@@ -365,14 +433,19 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	const AaGeom g = A->g;
 	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
-	// even step: blocks of 512 threads over several y rows at one block per
-	// CU, as the list kernel's even step (stream_shape)
-	const StreamShape es = stream_shape(&c, bs, (uint64_t)grid.x * bs * g.Y * g.X);
-	const unsigned rows = (unsigned)(es.block / bs);
-	const dim3 eblock((unsigned)bs, rows), egrid(grid.x, ((unsigned)g.Y + rows - 1) / rows, grid.z);
+	// even step: V cells per thread for the narrow types (-vec auto), blocks
+	// of 512 threads over several y rows at one block per CU, as the list
+	// kernel's even step (stream_shape)
+	const int vec = c.args.evenVec == 0 ? 1 : c.args.prec == PREC_FP32S ? 2 : c.args.prec == PREC_FP16S ? 4 : 1;
+	const int zq = (g.Z + vec - 1) / vec;
+	const int ebs = zq >= 256 ? 256 : ((zq + 63) / 64) * 64;
+	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
+	const StreamShape es = stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
+	const unsigned rows = (unsigned)(es.block / ebs);
+	const dim3 eblock((unsigned)ebs, rows), egrid(egx, ((unsigned)g.Y + rows - 1) / rows, grid.z);
 	// non-temporal accesses measured slower here than in the list kernel: off unless asked for
 	const bool nt = c.args.nontemporal == 1;
-	const AaLaunch L = {grid, block, egrid, eblock, es.lds, c.stream, A->dInfo, g, nt};
+	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto pair = [&](bool on, bool fl, bool fs, int it) {
 		switch (c.args.prec) {

@@ -182,7 +182,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			int k = 0;
 			while ((1 << k) < v) ++k;
 			a->f16Shift = k;
-		} else if (listKernel && (is("-vec") || is("--vec"))) {
+		} else if (is("-vec") || is("--vec")) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -vec requires auto or 1.");
 			const char * v = p->KernelArgs[++i];
 			if (!strcmp(v, "auto")) a->evenVec = 1;
@@ -219,8 +219,8 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>]\n");
-			if (listKernel) printf("  [-adj rle|full] [-vec auto|1] [-shadow eager|lazy] [-statistics]\n");
+			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1]\n");
+			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-statistics]\n");
 			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			return fail("kernel help requested.");

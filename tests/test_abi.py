@@ -139,7 +139,7 @@ def test_kernel_argument_errors(lbm, args, msg):
         lbm.Kernel("list-aa-pv-soa", geo, args)
 
 
-@pytest.mark.parametrize("args", [["-pad", "auto"], ["-adj", "full"], ["-vec", "1"], ["-shadow", "lazy"],
+@pytest.mark.parametrize("args", [["-pad", "auto"], ["-adj", "full"], ["-shadow", "lazy"],
                                   ["-odd-order", "band"], ["-statistics"]])
 def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
@@ -151,6 +151,7 @@ def test_aa_rejects_list_only_argument(lbm, args):
     (["-prec", "fp8"], "fp64, fp32s or fp16s"),
     (["-fp16-scale", "3000"], "power of two"),
     (["-even-cap", "maybe"], "on, off or force"),
+    (["-vec", "2"], "auto or 1"),
 ])
 def test_aa_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))

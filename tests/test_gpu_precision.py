@@ -199,7 +199,8 @@ def test_aa_shifted_bitexact(lbm, oracle, case, prec):
     omega, F = float(g["omega"]), float(g["xforce"])
     base = ["-prec", prec, "-lambda", repr(LAM)]
     want = oracle.aa_run_shifted(mask, dims, per, [60, 40], omega, F, prec, magic=LAM)
-    for extra in ([], ["-nt", "on"], ["-even-cap", "force"], ["-even-cap", "off", "-bs", "64"]):
+    for extra in ([], ["-vec", "1"], ["-nt", "on"], ["-even-cap", "force"], ["-even-cap", "force", "-vec", "1"],
+                  ["-even-cap", "off"]):
         print(f"aa case {case} -prec {prec} {extra}", flush=True)
         got = _aa_gpu(lbm, geo, [60, 40], omega, F, base + extra)
         assert np.array_equal(got, want), (extra, np.abs(got - want).max())
@@ -218,8 +219,9 @@ def test_aa_shifted_long_rows_and_nodes(lbm, oracle, prec):
     nodes = np.tile(oracle.W, (geo.n_fluid, 1)) + 1e-3 * rng.standard_normal((geo.n_fluid, 19))
     print(f"aa porous 24x20x300 -prec {prec}", flush=True)
     want = oracle.aa_run_shifted(m, geo.dims, geo.periodic, [40, 20], 1.25, 3e-5, prec, magic=LAM, nodes=nodes)
-    got = _aa_gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)], nodes=nodes)
-    assert np.array_equal(got, want)
+    for extra in ([], ["-vec", "1"]):
+        got = _aa_gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra, nodes=nodes)
+        assert np.array_equal(got, want), extra
     # the list kernel gives the same bits here (no open faces)
     got_list, _ = _gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)], nodes=nodes)
     assert np.array_equal(got_list, want)
@@ -229,3 +231,18 @@ def test_aa_shifted_long_rows_and_nodes(lbm, oracle, prec):
         got = _aa_gpu(lbm, geo, [30], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM), "-fp16-scale", "1024"],
                       nodes=nodes)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_aa_shifted_open_faces_ragged_rows(lbm, oracle, prec):
+    """All-fluid boxes whose faces are open (ghost layer in use) and whose z
+    rows are not a multiple of the even step's cells per thread: the groups
+    that reach past the row end write the ghost values back unchanged."""
+    for dims, per in (((9, 7, 7), (0, 0, 0)), ((11, 6, 10), (1, 0, 0)), ((5, 8, 13), (0, 1, 0))):
+        mask = np.ones(int(np.prod(dims)), dtype=np.int32)
+        geo = lbm.Geometry.from_mask(mask, dims, per)
+        want = oracle.aa_run_shifted(mask, dims, per, [20, 10], 1.4, 5e-5, prec, magic=LAM)
+        for extra in ([], ["-vec", "1"], ["-even-cap", "force"]):
+            print(f"aa open {dims} {per} -prec {prec} {extra}", flush=True)
+            got = _aa_gpu(lbm, geo, [20, 10], 1.4, 5e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
+            assert np.array_equal(got, want), (dims, extra)
