@@ -150,7 +150,10 @@ int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out);
  * device: mass = sum_n sum_d f_d and momentumX = sum_n sum_d c_x,d f_d
  * (node views; the counterparts of the per-node loops of KernelDensity and
  * KernelStatisticsAdv, src/Kernel.c:209-254, 633-755).  Valid between
- * Kernel() calls; pending SetNode changes are uploaded first. */
+ * Kernel() calls; pending SetNode changes are uploaded first.
+ * LbmHipMoments takes either kernel (list-aa-pv-soa or aa-soa);
+ * LbmHipListMoments only the list kernel. */
+int LbmHipMoments(KernelData * kd, double * mass, double * momentumX);
 int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX);
 
 /* Runs the case in Kernel() calls of checkEvery steps (rounded up to even)
@@ -158,15 +161,17 @@ int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX);
  * or maxSteps steps have run.  The host shadow is refreshed once at the end
  * (when shadow download is on); kd->Duration is the summed step time.
  * No counterpart in the reference, which always runs MaxIterations steps
- * (src/Main.c:574). */
+ * (src/Main.c:574).  LbmHipRunSteady takes either kernel,
+ * LbmHipListRunSteady only the list kernel. */
 typedef struct {
 	int steps;          /* steps run */
 	int converged;      /* 1 when the tolerance was met */
 	double relChange;   /* last relative change of momentumX (-1 before two checks) */
-	double mass;        /* LbmHipListMoments after the last chunk */
+	double mass;        /* LbmHipMoments after the last chunk */
 	double momentumX;
 	double seconds;     /* summed step-loop time */
 } LbmHipSteady;
+int LbmHipRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out);
 int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out);
 
 /* ---- batch: many list-aa-pv-soa lattices stepped together --------------

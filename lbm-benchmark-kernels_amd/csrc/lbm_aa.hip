@@ -54,6 +54,8 @@ struct AaData {
 	void * hPdf = nullptr;
 	uint32_t * hInfo = nullptr;
 	size_t nPdf = 0;
+	double * dMoments = nullptr;  // per-block pairs, then the total (allocated at first use)
+	uint32_t momentBlocks = 0;
 };
 
 static AaData * as_aa(KernelData * kd) { return reinterpret_cast<AaData *>(kd); }
@@ -340,6 +342,43 @@ static void aa_bc_set(KernelData * kd, int x, int y, int z, int dir, PdfT pdf)
 	A->c.shadowDirty = 1;
 }
 
+// Cell sums in the local layout (after a completed Kernel() call), in FP64:
+// per block of 256 cells the sum over its fluid cells of sum_d v_d and of
+// sum_d c_x,d v_d (v the stored value); k_sum_pairs then adds the block sums
+// in a fixed order.
+template <class Codec>
+__global__ __launch_bounds__(256) void k_aa_moments(const typename Codec::S * __restrict__ pdf, Codec cod,
+                                                    const uint32_t * __restrict__ info, AaGeom g,
+                                                    double * __restrict__ partial)
+{
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	double m = 0.0, jx = 0.0;
+	if (i < (uint32_t)g.X * g.Y * g.Z && (info[i] & kInfoFluid)) {
+		const int z = (int)(i % g.Z), y = (int)((i / g.Z) % g.Y), x = (int)(i / ((uint32_t)g.Z * g.Y));
+		const uint64_t base = aa_slot(g, 0, x, y, z);
+		for (int d = 0; d < 19; ++d) {
+			const double v = (double)cod.template load<false>(&pdf[base + (uint64_t)d * g.plane]);
+			m += v;
+			jx += cx(d) * v;
+		}
+	}
+	__shared__ double sm[256], sj[256];
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		partial[2 * blockIdx.x] = sm[0];
+		partial[2 * blockIdx.x + 1] = sj[0];
+	}
+}
+
 // One even + odd step pair.  ev: four events around the two launches
 // (per-launch timing) or null.
 struct AaLaunch {
@@ -400,35 +439,37 @@ static bool aa_download(KernelData * kd)
 	return true;
 }
 
-static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+static bool aa_upload(AaData * A)
 {
-	(void)ld;
+	Common & c = A->c;
+	if (!c.shadowDirty) { c.t.uploadMs = 0.0; return true; }
+	const double t0 = (double)clock() / CLOCKS_PER_SEC;
+	if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, aa_bytes(A), hipMemcpyHostToDevice, c.stream), "upload of the host lattice")) return false;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "hipStreamSynchronize")) return false;
+	c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
+	c.shadowDirty = 0;
+	return true;
+}
+
+static bool aa_run(KernelData * kd, const CaseData * cd)
+{
 	AaData * A = as_aa(kd);
 	Common & c = A->c;
 	if (cd->VtkOutput) {
 		// the reference's aa-soa writes some snapshots between the even and
 		// the odd step, in the swapped layout (src/BenchKernelD3Q19Aa.c:319-331)
-		fail("aa-soa: VTK output during the run is built for list-aa-pv-soa only.");
-		return;
+		return fail("aa-soa: VTK output during the run is built for list-aa-pv-soa only.");
 	}
 	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
-	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) { fail("omega must be in (0, 2), got %f.", cd->Omega); return; }
+	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) return fail("omega must be in (0, 2), got %f.", cd->Omega);
 
-	if (c.shadowDirty) {
-		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!hip_ok(hipMemcpyAsync(A->dPdf, A->hPdf, aa_bytes(A), hipMemcpyHostToDevice, c.stream), "upload of the host lattice")) return;
-		if (!hip_ok(hipStreamSynchronize(c.stream), "hipStreamSynchronize")) return;
-		c.t.uploadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
-		c.shadowDirty = 0;
-	} else {
-		c.t.uploadMs = 0.0;
-	}
+	if (!aa_upload(A)) return false;
 
 	const TrtConsts k = trt_consts(cd->Omega, c.args.magic);
 	const ForceTerms ft = force_terms(cd->XForce);
 	const bool force = c.args.forceOn != 0;
 	const bool perLaunch = c.launchEvents != 0;
-	if (perLaunch && !common_events(&c, steps)) return;
+	if (perLaunch && !common_events(&c, steps)) return false;
 
 	const AaGeom g = A->g;
 	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
@@ -458,14 +499,14 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	};
 
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
-	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return;
+	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
 	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
 		const bool last = it + 2 >= steps;
 		pair(force, force && it > 0, force && last, it);
 	}
-	if (!hip_ok(hipGetLastError(), "step kernel launch")) return;
-	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return;
-	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return;
+	if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
+	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return false;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "step kernels")) return false;
 
 	common_collect(&c, A->nFluid > 0 ? steps : 0, perLaunch);
 	c.t.steps = steps;
@@ -475,19 +516,78 @@ static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 	c.shadowStale = A->nFluid > 0 && steps > 0;
 	if (c.shadowDownload) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!aa_download(kd)) return;
+		if (!aa_download(kd)) return false;
 		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
 	}
+	return true;
 }
+
+static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
+{
+	(void)ld;
+	(void)aa_run(kd, cd);
+}
+
 
 Common * aa_common(KernelData * kd)
 {
 	return (kd && kd->Kernel == aa_kernel) ? &as_aa(kd)->c : nullptr;
 }
 
+bool aa_steps(KernelData * kd, const CaseData * cd)
+{
+	if (!aa_common(kd)) return fail("not an aa-soa kernel.");
+	return aa_run(kd, cd);
+}
+
+bool aa_download_shadow(KernelData * kd)
+{
+	if (!aa_common(kd)) return fail("not an aa-soa kernel.");
+	return aa_download(kd);
+}
+
+// Mass and x momentum of the device lattice in FP64 (shifted storage: the
+// stored deviations plus the weights, which add 1 per fluid cell to the
+// mass and nothing to the momentum).
+bool aa_moments(KernelData * kd, double * mass, double * jx)
+{
+	if (!aa_common(kd)) return fail("not an aa-soa kernel.");
+	AaData * A = as_aa(kd);
+	Common & c = A->c;
+	*mass = 0.0;
+	*jx = 0.0;
+	if (!aa_upload(A)) return false;
+	if (A->nFluid == 0) return true;
+	const AaGeom g = A->g;
+	const uint32_t nb = (uint32_t)(((uint64_t)g.X * g.Y * g.Z + 255) / 256);
+	if (!A->dMoments) {
+		if (!hip_ok(hipMalloc((void **)&A->dMoments, sizeof(double) * 2 * ((size_t)nb + 1)), "allocating moment sums")) return false;
+		A->momentBlocks = nb;
+	}
+	double * part = A->dMoments, * sum = A->dMoments + 2 * (size_t)nb;
+	const float f16s = (float)(1 << c.args.f16Shift);
+	(void)hipGetLastError();
+	switch (c.args.prec) {
+		case PREC_FP32S: k_aa_moments<<<nb, 256, 0, c.stream>>>((const float *)A->dPdf, CodecF32{}, A->dInfo, g, part); break;
+		case PREC_FP16S:
+			k_aa_moments<<<nb, 256, 0, c.stream>>>((const uint16_t *)A->dPdf, CodecF16{f16s, 1.0f / f16s}, A->dInfo, g, part);
+			break;
+		default: k_aa_moments<<<nb, 256, 0, c.stream>>>((const double *)A->dPdf, CodecF64{}, A->dInfo, g, part);
+	}
+	k_sum_pairs<256><<<1, 256, 0, c.stream>>>(part, nb, sum);
+	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
+	double h[2];
+	if (!hip_ok(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, c.stream), "moment download")) return false;
+	if (!hip_ok(hipStreamSynchronize(c.stream), "moment kernels")) return false;
+	*mass = h[0] + (c.args.prec == PREC_FP64 ? 0.0 : (double)A->nFluid);
+	*jx = h[1];
+	return true;
+}
+
 static void aa_release(AaData * A)
 {
 	if (!A) return;
+	(void)hipFree(A->dMoments);
 	(void)hipFree(A->dInfo);
 	(void)hipFree(A->dPdf);
 	host_free(A->hPdf);

@@ -54,6 +54,40 @@ int list_even_vec(const ListData * L);
 // Lattice weights w_d in direction order.
 extern const double kW[19];
 
+// Adds nBlocks (mass, x momentum) pairs in a fixed order (one block of B
+// threads): the second pass of the FP64 moment sums.
+template <int B>
+__global__ __launch_bounds__(B) void k_sum_pairs(const double * __restrict__ partial, uint32_t nBlocks,
+                                                 double * __restrict__ out)
+{
+	__shared__ double sm[B], sj[B];
+	double m = 0.0, jx = 0.0;
+	for (uint32_t i = threadIdx.x; i < nBlocks; i += B) {
+		m += partial[2 * i];
+		jx += partial[2 * i + 1];
+	}
+	sm[threadIdx.x] = m;
+	sj[threadIdx.x] = jx;
+	__syncthreads();
+	for (int off = B / 2; off > 0; off >>= 1) {
+		if ((int)threadIdx.x < off) {
+			sm[threadIdx.x] += sm[threadIdx.x + off];
+			sj[threadIdx.x] += sj[threadIdx.x + off];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		out[0] = sm[0];
+		out[1] = sj[0];
+	}
+}
+
+// aa-soa counterparts of the list kernel's moment sums and step runs
+// (lbm_aa.hip): false with the error recorded.
+bool aa_moments(KernelData * kd, double * mass, double * jx);
+bool aa_steps(KernelData * kd, const CaseData * cd);
+bool aa_download_shadow(KernelData * kd);
+
 // Adjacency of node n for the odd step, from the full table or the runs.
 struct AdjFull {
 	const uint32_t * adj;

@@ -328,30 +328,6 @@ __global__ __launch_bounds__(256) void k_list_moments(const typename Codec::S * 
 	}
 }
 
-__global__ __launch_bounds__(256) void k_sum_pairs(const double * __restrict__ partial, uint32_t nBlocks, double * __restrict__ out)
-{
-	__shared__ double sm[256], sj[256];
-	double m = 0.0, jx = 0.0;
-	for (uint32_t i = threadIdx.x; i < nBlocks; i += 256) {
-		m += partial[2 * i];
-		jx += partial[2 * i + 1];
-	}
-	sm[threadIdx.x] = m;
-	sj[threadIdx.x] = jx;
-	__syncthreads();
-	for (int off = 128; off > 0; off >>= 1) {
-		if ((int)threadIdx.x < off) {
-			sm[threadIdx.x] += sm[threadIdx.x + off];
-			sj[threadIdx.x] += sj[threadIdx.x + off];
-		}
-		__syncthreads();
-	}
-	if (threadIdx.x == 0) {
-		out[0] = sm[0];
-		out[1] = sj[0];
-	}
-}
-
 // --------------------------------------------------------- node access
 // After any completed Kernel() call (always an even number of steps) and
 // before the first one, the lattice is in the local layout: node n's value
@@ -732,7 +708,7 @@ static bool list_moments(ListData * L, double * mass, double * jx)
 			break;
 		default: k_list_moments<<<nb, 256, 0, c.stream>>>((const double *)L->dPdf, CodecF64{}, nF, nC, part);
 	}
-	k_sum_pairs<<<1, 256, 0, c.stream>>>(part, nb, sum);
+	k_sum_pairs<256><<<1, 256, 0, c.stream>>>(part, nb, sum);
 	if (!hip_ok(hipGetLastError(), "moment kernels")) return false;
 	double h[2];
 	if (!hip_ok(hipMemcpyAsync(h, sum, sizeof(h), hipMemcpyDeviceToHost, c.stream), "moment download")) return false;
@@ -1021,25 +997,37 @@ int LbmHipListGetInfo(const KernelData * kd, LbmHipListInfo * out)
 	return 0;
 }
 
-int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX)
+}  // extern "C"
+
+namespace lbm {
+
+// Either kernel: moment sums, a chunk of steps, and the final download.
+static bool any_moments(KernelData * kd, double * m, double * j)
 {
-	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListMoments: not a list-aa-pv-soa kernel."); return -1; }
-	double m = 0.0, j = 0.0;
-	if (!list_moments(as_list(kd), &m, &j)) return -1;
-	if (mass) *mass = m;
-	if (momentumX) *momentumX = j;
-	return 0;
+	return list_data(kd) ? list_moments(list_data(kd), m, j) : aa_moments(kd, m, j);
 }
 
-int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out)
+static bool any_steps(KernelData * kd, const CaseData * cd)
 {
-	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListRunSteady: not a list-aa-pv-soa kernel."); return -1; }
+	return list_data(kd) ? list_steps(kd, cd, nullptr) : aa_steps(kd, cd);
+}
+
+static bool any_download(KernelData * kd)
+{
+	return list_data(kd) ? list_download(list_data(kd)) : aa_download_shadow(kd);
+}
+
+// Kernel() chunks of checkEvery steps (rounded up to even) until the
+// relative change of the x momentum between checks is at most tol, or
+// maxSteps steps ran.
+static int run_steady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out,
+                      const char * who)
+{
 	if (!cd || checkEvery <= 0 || maxSteps <= 0 || !(tol >= 0.0)) {
-		fail("LbmHipListRunSteady: needs a case, checkEvery > 0, maxSteps > 0 and tol >= 0.");
+		fail("%s: needs a case, checkEvery > 0, maxSteps > 0 and tol >= 0.", who);
 		return -1;
 	}
-	ListData * L = as_list(kd);
-	Common & c = L->c;
+	Common & c = *(list_data(kd) ? list_common(kd) : aa_common(kd));
 	const int every = checkEvery + (checkEvery & 1);
 	const int saveDownload = c.shadowDownload;
 	c.shadowDownload = 0;
@@ -1050,10 +1038,10 @@ int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double t
 	while (r.steps < maxSteps) {
 		CaseData chunk = *cd;
 		chunk.MaxIterations = every < maxSteps - r.steps ? every : maxSteps - r.steps;
-		if (!list_steps(kd, &chunk, nullptr)) { ok = false; break; }
+		if (!any_steps(kd, &chunk)) { ok = false; break; }
 		r.steps += chunk.MaxIterations + (chunk.MaxIterations & 1);
 		seconds += kd->Duration;
-		if (!list_moments(L, &r.mass, &r.momentumX)) { ok = false; break; }
+		if (!any_moments(kd, &r.mass, &r.momentumX)) { ok = false; break; }
 		if (havePrev) {
 			const double d = fabs(r.momentumX - prev), a = fabs(r.momentumX);
 			r.relChange = a > 0.0 ? d / a : (d > 0.0 ? INFINITY : 0.0);
@@ -1065,9 +1053,41 @@ int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double t
 	c.shadowDownload = saveDownload;
 	kd->Duration = seconds;
 	r.seconds = seconds;
-	if (ok && saveDownload && !list_download(L)) ok = false;
+	if (ok && saveDownload && !any_download(kd)) ok = false;
 	if (out) *out = r;
 	return ok ? 0 : -1;
+}
+
+}  // namespace lbm
+
+extern "C" {
+
+int LbmHipListMoments(KernelData * kd, double * mass, double * momentumX)
+{
+	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListMoments: not a list-aa-pv-soa kernel."); return -1; }
+	return LbmHipMoments(kd, mass, momentumX);
+}
+
+int LbmHipListRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out)
+{
+	if (!kd || kd->Kernel != list_kernel) { fail("LbmHipListRunSteady: not a list-aa-pv-soa kernel."); return -1; }
+	return run_steady(kd, cd, checkEvery, tol, maxSteps, out, "LbmHipListRunSteady");
+}
+
+int LbmHipMoments(KernelData * kd, double * mass, double * momentumX)
+{
+	if (!list_data(kd) && !aa_common(kd)) { fail("LbmHipMoments: not a kernel of this library."); return -1; }
+	double m = 0.0, j = 0.0;
+	if (!any_moments(kd, &m, &j)) return -1;
+	if (mass) *mass = m;
+	if (momentumX) *momentumX = j;
+	return 0;
+}
+
+int LbmHipRunSteady(KernelData * kd, CaseData * cd, int checkEvery, double tol, int maxSteps, LbmHipSteady * out)
+{
+	if (!list_data(kd) && !aa_common(kd)) { fail("LbmHipRunSteady: not a kernel of this library."); return -1; }
+	return run_steady(kd, cd, checkEvery, tol, maxSteps, out, "LbmHipRunSteady");
 }
 
 int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, uint32_t * coords)

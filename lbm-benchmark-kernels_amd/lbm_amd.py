@@ -121,7 +121,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipGetTimings", "LbmHipSetLaunchEvents", "LbmHipSetShadowDownload",
                "LbmHipSyncShadow", "LbmHipListGetInfo", "LbmHipListCopyBuild", "LbmHipDumpNodes",
                "LbmHipLoadNodes", "LbmHipDeviceInfo", "LbmHipStreamBench", "LbmHipListMoments",
-               "LbmHipListRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
+               "LbmHipListRunSteady", "LbmHipMoments", "LbmHipRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
                "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait"]
@@ -189,6 +189,7 @@ def hip_lib() -> C.CDLL:
         L.LbmHipDeviceInfo.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.LbmHipStreamBench.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(LbmHipStreamResult)]
         L.LbmHipListMoments.argtypes = [KD, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.LbmHipMoments.argtypes = [KD, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.LbmHipBatchCreate.argtypes = [C.POINTER(Parameters)]
         L.LbmHipBatchCreate.restype = C.c_void_p
         L.LbmHipBatchDestroy.argtypes = [C.c_void_p]
@@ -208,6 +209,8 @@ def hip_lib() -> C.CDLL:
         L.LbmHipBatchStats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.LbmHipListRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
                                           C.POINTER(LbmHipSteady)]
+        L.LbmHipRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
+                                      C.POINTER(LbmHipSteady)]
         _libs["hip"] = L
     return _libs["hip"]
 
@@ -409,7 +412,7 @@ class Kernel:
     def moments(self):
         """(mass, x momentum) of the device lattice, FP64 sums on the GPU."""
         m, j = C.c_double(), C.c_double()
-        if hip_lib().LbmHipListMoments(self.kd, C.byref(m), C.byref(j)) != 0:
+        if hip_lib().LbmHipMoments(self.kd, C.byref(m), C.byref(j)) != 0:
             raise RuntimeError(hip_lib().LbmHipLastError().decode())
         return m.value, j.value
 
@@ -417,7 +420,7 @@ class Kernel:
         r = LbmHipSteady()
         L = hip_lib()
         L.LbmHipClearError()
-        if L.LbmHipListRunSteady(self.kd, C.byref(cd), check_every, tol, max_steps, C.byref(r)) != 0:
+        if L.LbmHipRunSteady(self.kd, C.byref(cd), check_every, tol, max_steps, C.byref(r)) != 0:
             raise RuntimeError(L.LbmHipLastError().decode())
         return r
 

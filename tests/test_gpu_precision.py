@@ -246,3 +246,47 @@ def test_aa_shifted_open_faces_ragged_rows(lbm, oracle, prec):
             print(f"aa open {dims} {per} -prec {prec} {extra}", flush=True)
             got = _aa_gpu(lbm, geo, [20, 10], 1.4, 5e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
             assert np.array_equal(got, want), (dims, extra)
+
+
+@pytest.mark.parametrize("prec", ["fp64"] + PRECS)
+def test_aa_moments_match_node_sums(lbm, prec):
+    """aa-soa's FP64 device sums against sums of its node views, and
+    against the list kernel's on the same lattice (no open faces, so the two
+    kernels hold the same node views)."""
+    geo = lbm.Geometry.from_string("porous-3-0.75-9", (40, 36, 32))
+    args = ["-prec", prec, "-lambda", repr(LAM)]
+    with lbm.Kernel("aa-soa", geo, args) as k:
+        k.set_initial_weights()
+        m0, j0 = k.moments()
+        assert m0 == pytest.approx(geo.n_fluid, rel=1e-14) and abs(j0) < 1e-12
+        k.run(lbm.case_data(omega=1.2, xforce=1e-4, iterations=60))
+        m, j = k.moments()
+        d = k.dump()
+    assert m == pytest.approx(d.sum(), rel=1e-12)
+    assert j == pytest.approx((d @ CX).sum(), rel=1e-10)
+    assert j > 0
+    with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
+        k.set_initial_weights()
+        k.run(lbm.case_data(omega=1.2, xforce=1e-4, iterations=60))
+        ml, jl = k.moments()
+        assert np.array_equal(k.dump(), d)
+    assert ml == pytest.approx(m, rel=1e-13) and jl == pytest.approx(j, rel=1e-11)
+
+
+def test_aa_run_steady(lbm, oracle):
+    geo = lbm.Geometry.from_string("channel", (14, 12, 10), (1, 0, 0))
+    m = geo.mask.reshape(-1)
+    cd = lbm.case_data(omega=1.1, xforce=1e-4, iterations=0)
+    with lbm.Kernel("aa-soa", geo) as k:
+        k.set_initial_weights()
+        r = k.run_steady(cd, 15, 0.0, 100)
+        assert (r.steps, r.converged) == (100, 0)
+        assert (r.mass, r.momentumX) == k.moments()
+        got = k.dump()
+    want, _ = oracle.aa_run(m, geo.dims, geo.periodic, 100, 1.1, 1e-4)
+    assert np.array_equal(got, want)
+    with lbm.Kernel("aa-soa", geo, ["-prec", "fp32s"]) as k:
+        k.set_initial_weights()
+        r = k.run_steady(cd, 200, 1e-4, 40000)
+        assert r.converged == 1 and 0 <= r.relChange <= 1e-4 and r.steps % 200 == 0 and r.steps < 40000
+        assert r.seconds > 0 and k.duration == pytest.approx(r.seconds)
