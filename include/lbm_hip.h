@@ -19,7 +19,8 @@
  *                     reference's VERIFICATION build does (default on)
  *   -lambda <v>       TRT magic parameter (default 1/12, the value the
  *                     reference hard-codes, src/BenchKernelD3Q19ListAaPv.c:177)
- *   -bs <n>           threads per block of the step kernels (64..512, step 64)
+ *   -bs <n>           threads per block of the step kernels (64..512, step 64;
+ *                     list-aa-pv-soa -- aa-soa's blocks follow its z rows)
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
  *   -nt on|off        non-temporal loads/stores in the even step (default on
