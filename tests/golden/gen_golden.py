@@ -3,7 +3,8 @@
 Runs the reference's own list-aa-pv-soa and aa-soa kernels (compiled from
 its unmodified sources into oracle/_ref/ by oracle/Makefile, driven by the
 glue in oracle/ref_harness_*.c) on small geometries and on the two
-BASELINE-sized workloads, and stores inputs and outputs:
+full-size workloads (64^3 channel, 256^3 porous medium), and stores inputs
+and outputs:
 
   tiny_<case>.npz   full node views (nFluid x 19, mask order) after 100
                     steps, list build (adjacency node-major, grid, coords)
