@@ -50,6 +50,7 @@ struct KernelArgs {
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
 	int oddOrder = 2;      // list: odd-step block order, 0 plain, 1 XCD-grouped, 2 y bands per XCD
+	int launchEvents = 0;  // per-launch timing events (even/odd averages)
 };
 
 // PDF storage of the list kernel.
@@ -96,7 +97,7 @@ struct Common {
 	int iteration = -1;         // parity of the last completed step
 	int shadowDirty = 0;        // SetNode touched the host shadow since the last upload
 	int shadowStale = 0;        // the device lattice is newer than the host shadow
-	int launchEvents = 1;
+	int launchEvents = 0;       // from -launch-events; LbmHipSetLaunchEvents
 	int shadowDownload = 1;
 	hipEvent_t * ev = nullptr;  // 2 per launch
 	int nEv = 0;

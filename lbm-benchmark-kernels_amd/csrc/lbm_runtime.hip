@@ -203,6 +203,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			else return fail("argument -odd-order requires band, group or plain.");
 		} else if (listKernel && (is("-statistics") || is("--statistics"))) {
 			a->statistics = 1;
+		} else if (is("-launch-events") || is("--launch-events")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -launch-events requires on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "on")) a->launchEvents = 1;
+			else if (!strcmp(v, "off")) a->launchEvents = 0;
+			else return fail("argument -launch-events requires on or off.");
 		} else if (is("-quiet") || is("--quiet")) {
 			a->quiet = 1;
 		} else if (is("-even-cap") || is("--even-cap")) {
@@ -223,6 +229,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-statistics]\n");
 			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
+			printf("  [-launch-events on|off]\n");
 			return fail("kernel help requested.");
 		} else {
 			return fail("unknown kernel parameter: %s.", s);
@@ -310,6 +317,7 @@ bool shadow_writable(const Common * c)
 bool common_init(Common * c, const Parameters * params, bool listKernel)
 {
 	if (!parse_kernel_args(params, &c->args, listKernel)) return false;
+	c->launchEvents = c->args.launchEvents;
 	int n = 0;
 	const hipError_t e = hipGetDeviceCount(&n);
 	if (e == hipErrorNoDevice || (e == hipSuccess && n < 1))
@@ -358,7 +366,13 @@ bool common_events(Common * c, int launches)
 	if (!ev) return fail("allocating %d events failed.", need);
 	c->ev = ev;
 	for (int i = c->nEv; i < need; ++i) {
-		if (!hip_ok(hipEventCreate(&c->ev[i]), "hipEventCreate")) { c->nEv = i; return false; }
+		// timing only: no system-scope fence (its cache writeback and
+		// invalidation would otherwise slow the launch that follows; the
+		// host reads results only after a stream synchronisation)
+		if (!hip_ok(hipEventCreateWithFlags(&c->ev[i], hipEventDisableSystemFence), "hipEventCreateWithFlags")) {
+			c->nEv = i;
+			return false;
+		}
 	}
 	c->nEv = need;
 	return true;

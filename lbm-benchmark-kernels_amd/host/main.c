@@ -236,7 +236,8 @@ int main(int argc, char * argv[])
 	printf("#   MEM data vol.:     \t%.2f GByte\n", volGiB);
 	printf("#   MEM bandwidth:     \t%.2f GByte/s\n", volGiB / duration);
 	printf("#   performance:       \t%.3f MFLUP/s\n", perf);
-	printf("#   even step:         \t%.4f ms avg  odd step: %.4f ms avg\n", tm.evenMsAvg, tm.oddMsAvg);
+	if (tm.evenMsAvg > 0.0)   /* kernel argument -launch-events on */
+		printf("#   even step:         \t%.4f ms avg  odd step: %.4f ms avg\n", tm.evenMsAvg, tm.oddMsAvg);
 
 	printf("P:   %f MFLUP/s  t: %d  d: %f s  iter: %d  fnodes: %f x1e6  geo: %s  kernel: %s  %s  %s\n",
 	       perf, nThreads, duration, cd.MaxIterations, ld.nFluid / 1e6, geometry, kernelName,

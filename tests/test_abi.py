@@ -132,6 +132,7 @@ def test_init_fails_loudly_without_device(lbm):
     (["-vec", "2"], "auto or 1"),
     (["-even-cap", "maybe"], "on, off or force"),
     (["-odd-order", "zigzag"], "band, group or plain"),
+    (["-launch-events", "maybe"], "on or off"),
 ])
 def test_kernel_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))

@@ -192,6 +192,16 @@ def test_force_off_block_sizes_and_split_calls(lbm, oracle, name):
     got, _, _ = _run(lbm, name, geo, 40, 1.1, 1e-4, ["-force", "off"])
     want_off, _ = oracle.list_run(m, geo.dims, geo.periodic, 40, 1.1, 1e-4, vsize=1, force_on=False)
     assert np.array_equal(got, want_off)
+    # per-launch timing events: off by default, on by argument or setter; same results
+    got, _, t = _run(lbm, name, geo, 40, 1.1, 1e-4)
+    assert t.loopMs > 0 and t.evenMsAvg == 0 and t.oddMsAvg == 0
+    got, _, t = _run(lbm, name, geo, 40, 1.1, 1e-4, ["-launch-events", "on"])
+    assert np.array_equal(got, want) and t.evenMsAvg > 0 and t.oddMsAvg > 0
+    with lbm.Kernel(name, geo) as k:
+        k.set_initial_weights()
+        k.set_launch_events(True)
+        k.run(lbm.case_data(omega=1.1, xforce=1e-4, iterations=40))
+        assert k.timings().oddMsAvg > 0 and np.array_equal(k.dump(), want)
 
 
 def test_edge_geometries(lbm, oracle):
