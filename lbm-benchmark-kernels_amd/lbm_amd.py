@@ -98,7 +98,7 @@ class LbmHipBatchStatus(C.Structure):
 
 class LbmHipBatchTuning(C.Structure):
     _fields_ = [("evenBlock", C.c_int), ("evenCap", C.c_int), ("vec", C.c_int), ("oddBlock", C.c_int),
-                ("evenMs", C.c_double), ("oddMs", C.c_double)]
+                ("evenMs", C.c_double), ("oddMs", C.c_double), ("groupMB", C.c_int), ("pairMs", C.c_double)]
 
 
 class LbmHipStreamResult(C.Structure):
@@ -244,14 +244,15 @@ def device_info():
 
 # --------------------------------------------------------------------------
 class Geometry:
-    """A LatticeDesc owned by Python.  ``mask`` is a (nZ, nY, nX) view."""
+    """A LatticeDesc whose mask lives in a numpy array.  ``mask`` is a
+    (nZ, nY, nX) view of that array and keeps it alive on its own, so it stays
+    valid after the Geometry is gone."""
 
-    def __init__(self, ld: LatticeDesc, keep=None, owned_by_c: bool = False):
+    def __init__(self, ld: LatticeDesc, keep: np.ndarray):
         self.ld = ld
         self._keep = keep
-        self._owned_by_c = owned_by_c
         nx, ny, nz = self.dims
-        self.mask = np.ctypeslib.as_array(ld.Lattice, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
+        self.mask = keep.reshape(nz, ny, nx)
 
     @property
     def dims(self):
@@ -267,11 +268,16 @@ class Geometry:
 
     @classmethod
     def from_string(cls, gtype: str, dims: Sequence[int], periodic=(0, 0, 0)) -> "Geometry":
-        """GeoCreateByStr of liblbmhost.so (src/Geometry.c:37 semantics)."""
+        """GeoCreateByStr of liblbmhost.so (src/Geometry.c:37 semantics); the
+        mask is copied into numpy memory and the C allocation freed."""
         ld = LatticeDesc()
         host_lib().GeoCreateByStr(gtype.encode(), (C.c_int * 3)(*dims), (C.c_int * 3)(*periodic),
                                   C.byref(ld))
-        return cls(ld, owned_by_c=True)
+        n = int(ld.Dims[0]) * int(ld.Dims[1]) * int(ld.Dims[2])
+        m = np.ctypeslib.as_array(ld.Lattice, shape=(n,)).copy()
+        host_lib().GeoFree(C.byref(ld))
+        ld.Lattice = m.ctypes.data_as(C.POINTER(C.c_int))
+        return cls(ld, m)
 
     @classmethod
     def from_mask(cls, mask: np.ndarray, dims: Sequence[int], periodic=(0, 0, 0), name=b"mask"):
@@ -287,14 +293,7 @@ class Geometry:
         ld.nOutlet = int(np.count_nonzero(m == 4))
         ld.PeriodicX, ld.PeriodicY, ld.PeriodicZ = (int(p) for p in periodic)
         ld.Name = name
-        return cls(ld, keep=m)
-
-    def __del__(self):
-        try:
-            if self._owned_by_c and self.ld.Lattice:
-                host_lib().GeoFree(C.byref(self.ld))
-        except Exception:
-            pass
+        return cls(ld, m)
 
 
 def case_data(omega=1.0, xforce=1e-5, iterations=10, rho_in=1.0, rho_out=1.0) -> CaseData:
@@ -519,8 +518,8 @@ class Batch:
         self._check(hip_lib().LbmHipBatchGetTuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, even_block: int, even_cap: int, vec: int, odd_block: int):
-        t = LbmHipBatchTuning(even_block, even_cap, vec, odd_block, 0.0, 0.0)
+    def set_tuning(self, even_block: int, even_cap: int, vec: int, odd_block: int, group_mb: int = 0):
+        t = LbmHipBatchTuning(even_block, even_cap, vec, odd_block, 0.0, 0.0, group_mb, 0.0)
         self._check(hip_lib().LbmHipBatchSetTuning(self.h, C.byref(t)))
 
     def autotune(self, pairs: int = 4) -> LbmHipBatchTuning:

@@ -297,6 +297,6 @@ def test_index_range_limit(lbm, name):
     ld.Dims = (lbm.C.c_int * 3)(1000, 1000, 300)
     ld.nCells, ld.nFluid = 300_000_000, 300_000_000
     geo = lbm.Geometry.__new__(lbm.Geometry)
-    geo.ld, geo._owned_by_c, geo._keep = ld, False, None
+    geo.ld, geo._keep = ld, None
     with pytest.raises(RuntimeError, match="2\\^32"):
         lbm.Kernel(name, geo)

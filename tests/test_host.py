@@ -100,3 +100,14 @@ def test_driver_without_device(tmp_path, lbm):
     r = _driver("-verify", cwd=tmp_path)
     assert r.returncode == 1
     assert "no HIP device" in r.stdout
+
+
+def test_mask_outlives_geometry(lbm):
+    # the mask view keeps its memory alive after the Geometry object is gone
+    import gc
+    m = lbm.Geometry.from_string("channel", (12, 10, 8), (1, 0, 0)).mask
+    gc.collect()
+    junk = [lbm.Geometry.from_string("box", (12, 10, 8)) for _ in range(20)]   # reuse freed memory
+    want = lbm.Geometry.from_string("channel", (12, 10, 8), (1, 0, 0)).mask
+    assert np.array_equal(m, want) and m.shape == (8, 10, 12)
+    del junk
