@@ -211,6 +211,12 @@ typedef struct {
 	int vec;            /* nodes per thread of the even step: 1, 2 (fp32s), 4 (fp16s) */
 	int oddBlock;       /* threads per block of the odd step (64..512) */
 	double evenMs, oddMs;   /* per launch, as measured by LbmHipBatchAutotune */
+	int groupMB;        /* > 0: domains stepped in groups of about this many MiB,
+	                       each group through a whole interval before the next
+	                       (its lattices stay in the last-level cache; default
+	                       256); 0, or all domains within one group: every
+	                       launch covers all domains (replayed from a graph) */
+	double pairMs;      /* per step pair with the chosen groups (autotune) */
 } LbmHipBatchTuning;
 /* params: kernel arguments shared by all domains (-prec, -lambda, -adj, ...). */
 LbmHipBatch * LbmHipBatchCreate(const Parameters * params);
@@ -237,7 +243,9 @@ int LbmHipBatchSync(LbmHipBatch * b, int id);
 int LbmHipBatchGetTuning(LbmHipBatch * b, LbmHipBatchTuning * out);
 int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
 /* Times `pairs` launches of every candidate shape on the active domains and
- * keeps the fastest; the lattices are restored afterwards. */
+ * keeps the fastest, then `pairs` step pairs for groups of 0 (none), 64,
+ * 128, 192, 256, 384 and 512 MiB with that shape; the lattices are restored
+ * afterwards. */
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */
 int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);

@@ -62,14 +62,17 @@ def test_batch_launch_shapes(lbm, prec):
     geos = _geos(lbm)[:3]
     vmax = {"fp64": 1, "fp32s": 2, "fp16s": 4}[prec]
     want = [_alone(lbm, g, om, F, [40] if prec != "fp16s" else [20, 20], args) for g, om, F in geos]
-    shapes = [(512, 1, vmax, 256), (256, 0, 1, 128), (512, 0, vmax, 512), (64, 0, 1, 64), (256, 1, vmax, 256)]
+    # fifth entry: groups of about that many MiB (one or two of these domains each)
+    shapes = [(512, 1, vmax, 256), (256, 0, 1, 128), (512, 0, vmax, 512), (64, 0, 1, 64), (256, 1, vmax, 256),
+              (512, 1, vmax, 256, 1), (256, 0, 1, 128, 2)]
     for shape in shapes + ["auto"]:
         print(f"batch {prec} shape {shape}", flush=True)
         with lbm.Batch(args + ["-adj", "full"] if shape == (64, 0, 1, 64) else args) as b:
             ids = [b.add(g, om, F) for g, om, F in geos]
             if shape == "auto":
                 t = b.autotune(2)
-                assert t.evenMs > 0 and t.oddMs > 0 and 1 <= t.vec <= vmax
+                assert t.evenMs > 0 and t.oddMs > 0 and t.pairMs > 0 and 1 <= t.vec <= vmax
+                assert t.groupMB in (0, 64, 128, 192, 256, 384, 512)
             else:
                 b.set_tuning(*shape)
             b.step(20)
@@ -78,10 +81,14 @@ def test_batch_launch_shapes(lbm, prec):
                 assert np.array_equal(b.dump(i), w), (shape, i)
 
 
-def test_batch_membership_and_convergence(lbm):
+@pytest.mark.parametrize("group_mb", [0, 1])
+def test_batch_membership_and_convergence(lbm, group_mb):
     args = ["-lambda", repr(LAM)]
     geos = _geos(lbm)
     with lbm.Batch(args) as b:
+        if group_mb:
+            b.set_tuning(512, 1, 1, 256, group_mb)      # about one domain per group
+            assert b.tuning().groupMB == group_mb
         a = b.add(*geos[0], max_steps=40)
         c = b.add(*geos[1], max_steps=0)
         d = b.add(*geos[2], max_steps=100)
