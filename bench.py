@@ -117,7 +117,7 @@ def batch_mode(args, lbm):
             setup = time.perf_counter() - t0
             tune = b.autotune(4)
             _log(f"{prec}: {len(ids)} domains set up in {setup:.1f} s; tuning even {tune.evenBlock}/cap "
-                 f"{tune.evenCap}/vec {tune.vec} odd {tune.oddBlock} groups {tune.groupMB} MiB "
+                 f"{tune.evenCap}/vec {tune.vec}/nt {tune.nontemporal} odd {tune.oddBlock} groups {tune.groupMB} MiB "
                  f"(pair {tune.pairMs:.3f} ms)")
             t0 = time.perf_counter()
             active, rounds = len(ids), 0
@@ -139,7 +139,7 @@ def batch_mode(args, lbm):
                          "tuning": {"even_block": tune.evenBlock, "even_cap": tune.evenCap, "vec": tune.vec,
                                     "odd_block": tune.oddBlock, "even_ms": round(tune.evenMs, 4),
                                     "odd_ms": round(tune.oddMs, 4), "group_mib": tune.groupMB,
-                                    "pair_ms": round(tune.pairMs, 4)},
+                                    "pair_ms": round(tune.pairMs, 4), "nontemporal": tune.nontemporal},
                          "_perm": perm, "_steps": [x.steps for x in st]}
             _log(f"{prec}: {res[prec]['solves_per_s']} solves/s, {res[prec]['mflups']} MFLUP/s, "
                  f"{res[prec]['converged']} converged")

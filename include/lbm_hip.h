@@ -217,6 +217,7 @@ typedef struct {
 	                       256); 0, or all domains within one group: every
 	                       launch covers all domains (replayed from a graph) */
 	double pairMs;      /* per step pair with the chosen groups (autotune) */
+	int nontemporal;    /* 1: non-temporal loads/stores in the even step (-nt) */
 } LbmHipBatchTuning;
 /* params: kernel arguments shared by all domains (-prec, -lambda, -adj, ...). */
 LbmHipBatch * LbmHipBatchCreate(const Parameters * params);

@@ -464,7 +464,7 @@ static void launch_batch_even(Batch * b, const ChunkRange & c, Codec cod, bool f
 	using R = typename Codec::R;
 	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
 	const dim3 g(c.evenGrid), t(b->tune.evenBlock);
-	const bool nt = b->args.nontemporal != 0;
+	const bool nt = b->tune.nontemporal != 0;
 	if (fl && nt)  k_batch_even<Codec, V, true, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
 	else if (fl)   k_batch_even<Codec, V, true, false><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
 	else if (nt)   k_batch_even<Codec, V, false, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
@@ -739,6 +739,7 @@ LbmHipBatch * LbmHipBatchCreate(const Parameters * params)
 	b->tune.vec = b->args.evenVec ? vec_max(b->args.prec) : 1;
 	b->tune.oddBlock = b->args.blockSize;
 	b->tune.groupMB = kDefaultGroupMB;
+	b->tune.nontemporal = b->args.nontemporal != 0;   // -nt (default on)
 	return reinterpret_cast<LbmHipBatch *>(b);
 }
 
@@ -912,6 +913,7 @@ int LbmHipBatchSetTuning(LbmHipBatch * handle, const LbmHipBatchTuning * t)
 	}
 	b->tune = *t;
 	b->tune.evenCap = t->evenCap != 0;
+	b->tune.nontemporal = t->nontemporal != 0;
 	b->tablesDirty = true;
 	b->graphDirty = true;
 	return 0;
@@ -1047,6 +1049,15 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 			float ms = 0.0f;
 			if (!(ok = time_it(false, false, &ms))) break;
 			if (ms < bestPair) { bestPair = ms; best.oddBlock = o; }
+		}
+		// non-temporal even-step accesses stream past the cache the groups rely on
+		for (int nt = 0; nt <= 1 && ok; ++nt) {
+			b->tune = best;
+			b->tune.nontemporal = nt;
+			b->tablesDirty = true;
+			float ms = 0.0f;
+			if (!(ok = time_it(false, false, &ms))) break;
+			if (ms < bestPair) { bestPair = ms; best.nontemporal = nt; }
 		}
 	}
 	b->tune = ok ? best : saved;

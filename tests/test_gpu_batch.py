@@ -64,7 +64,7 @@ def test_batch_launch_shapes(lbm, prec):
     want = [_alone(lbm, g, om, F, [40] if prec != "fp16s" else [20, 20], args) for g, om, F in geos]
     # fifth entry: groups of about that many MiB (one or two of these domains each)
     shapes = [(512, 1, vmax, 256), (256, 0, 1, 128), (512, 0, vmax, 512), (64, 0, 1, 64), (256, 1, vmax, 256),
-              (512, 1, vmax, 256, 1), (256, 0, 1, 128, 2)]
+              (512, 1, vmax, 256, 1), (256, 0, 1, 128, 2, 0), (512, 1, vmax, 256, 0, 0)]
     for shape in shapes + ["auto"]:
         print(f"batch {prec} shape {shape}", flush=True)
         with lbm.Batch(args + ["-adj", "full"] if shape == (64, 0, 1, 64) else args) as b:
