@@ -224,6 +224,11 @@ int main(int argc, char * argv[])
 	const double volGiB = loopBalance * ld.nFluid * cd.MaxIterations / 1024. / 1024. / 1024.;
 
 	kf->Deinit(&ld, &kd);
+	/* storage tag of the P: line: the reference's PRECISION ("dp"), or the
+	 * -prec kernel argument */
+	const char * precTag = "dp";
+	for (int j = 0; j + 1 < p.nKernelArgs; ++j)
+		if (!strcmp(p.KernelArgs[j], "-prec") || !strcmp(p.KernelArgs[j], "--prec")) precTag = p.KernelArgs[j + 1];
 	free(kargs);
 
 	const double perf = (double)ld.nFluid * (double)cd.MaxIterations / duration / 1.e6;
@@ -241,7 +246,7 @@ int main(int argc, char * argv[])
 
 	printf("P:   %f MFLUP/s  t: %d  d: %f s  iter: %d  fnodes: %f x1e6  geo: %s  kernel: %s  %s  %s\n",
 	       perf, nThreads, duration, cd.MaxIterations, ld.nFluid / 1e6, geometry, kernelName,
-	       "VERIFICATION", "dp");
+	       "VERIFICATION", precTag);
 
 	int rc = 0;
 	if (verify) {
