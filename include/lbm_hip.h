@@ -24,11 +24,13 @@
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
  *   -nt on|off        non-temporal loads/stores in the even step (default on
- *                     for list-aa-pv-soa, off for aa-soa: measured faster)
+ *                     for list-aa-pv-soa; for aa-soa off, except shifted
+ *                     storage on lattices over 256 MiB: measured faster)
  *   -even-cap on|off|force
  *                     even step in 512-thread blocks, one block per compute
  *                     unit (default on for lattices of at least 16 such
- *                     blocks per compute unit; force: at any size; -bs then
+ *                     blocks per compute unit, except aa-soa with shifted
+ *                     storage over 256 MiB; force: at any size; -bs then
  *                     applies to the odd step)
  *   -prec fp64|fp32s|fp16s
  *                     PDF storage (both kernels).  fp64 (default) is the

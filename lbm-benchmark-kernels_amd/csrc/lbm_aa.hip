@@ -481,11 +481,17 @@ static bool aa_run(KernelData * kd, const CaseData * cd)
 	const int zq = (g.Z + vec - 1) / vec;
 	const int ebs = zq >= 256 ? 256 : ((zq + 63) / 64) * 64;
 	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
-	const StreamShape es = stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
+	// Defaults measured at 64^3 and 256^3 (duct channel): FP64 runs best with
+	// plain accesses and the occupancy cap; the narrow types on lattices larger
+	// than the 256 MiB last-level cache with non-temporal accesses and full
+	// occupancy (+8 %), on smaller ones like FP64 (non-temporal accesses would
+	// push the cache-resident lattice out).
+	const bool wideDefault = c.args.prec != PREC_FP64 && aa_bytes(A) > (size_t)256 << 20;
+	const StreamShape es = (wideDefault && !c.args.evenCapGiven)
+	                       ? StreamShape{ebs, 0} : stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
 	const unsigned rows = (unsigned)(es.block / ebs);
 	const dim3 eblock((unsigned)ebs, rows), egrid(egx, ((unsigned)g.Y + rows - 1) / rows, grid.z);
-	// non-temporal accesses measured slower here than in the list kernel: off unless asked for
-	const bool nt = c.args.nontemporal == 1;
+	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && wideDefault);
 	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto pair = [&](bool on, bool fl, bool fs, int it) {

@@ -43,6 +43,7 @@ struct KernelArgs {
 	int adjRle = 1;        // list: odd step reads run-compressed adjacency
 	int nontemporal = -1;  // non-temporal loads/stores in the even step: 1 on, 0 off, -1 the kernel's default
 	int evenCap = 1;       // even step: 512-thread blocks, one block per CU (2: also on small lattices)
+	int evenCapGiven = 0;  // -even-cap was on the command line (else the kernel's default applies)
 	int prec = 0;          // list: PDF storage, see enum Prec
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
 	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step

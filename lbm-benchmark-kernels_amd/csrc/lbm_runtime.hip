@@ -214,6 +214,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 		} else if (is("-even-cap") || is("--even-cap")) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -even-cap requires on, off or force.");
 			const char * v = p->KernelArgs[++i];
+			a->evenCapGiven = 1;
 			if (!strcmp(v, "on")) a->evenCap = 1;
 			else if (!strcmp(v, "off")) a->evenCap = 0;
 			else if (!strcmp(v, "force")) a->evenCap = 2;

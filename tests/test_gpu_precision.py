@@ -290,3 +290,16 @@ def test_aa_run_steady(lbm, oracle):
         r = k.run_steady(cd, 200, 1e-4, 40000)
         assert r.converged == 1 and 0 <= r.relChange <= 1e-4 and r.steps % 200 == 0 and r.steps < 40000
         assert r.seconds > 0 and k.duration == pytest.approx(r.seconds)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_aa_shifted_large_lattice_defaults(lbm, prec):
+    """Over 256 MiB of shifted storage aa-soa's even step defaults to
+    non-temporal accesses at full occupancy: the same bits as the explicit
+    cache-friendly launch."""
+    geo = lbm.Geometry.from_string("channel", (80, 256, 256), (1, 0, 0))   # 309 MiB in FP16, 514 in FP32
+    base = ["-prec", prec, "-lambda", repr(LAM)]
+    print(f"aa 80x256x256 -prec {prec} defaults vs explicit", flush=True)
+    got = _aa_gpu(lbm, geo, [20], 1.2, 1e-5, base)
+    want = _aa_gpu(lbm, geo, [20], 1.2, 1e-5, base + ["-nt", "off", "-even-cap", "on"])
+    assert np.array_equal(got, want)
