@@ -29,7 +29,8 @@
  *   -even-cap on|off|force
  *                     even step in 512-thread blocks, one block per compute
  *                     unit (default on for lattices of at least 16 such
- *                     blocks per compute unit, except aa-soa with shifted
+ *                     blocks per compute unit, except list-aa-pv-soa with
+ *                     four FP16 nodes per thread and aa-soa with shifted
  *                     storage over 256 MiB; force: at any size; -bs then
  *                     applies to the odd step)
  *   -prec fp64|fp32s|fp16s

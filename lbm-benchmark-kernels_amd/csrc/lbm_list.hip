@@ -607,7 +607,10 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
 	const int vec = list_even_vec(L);
-	const StreamShape es = stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
+	// four FP16 nodes per thread run faster at full occupancy (measured +6 % at
+	// 256^3 porous); the other storage types with the occupancy cap
+	const StreamShape es = (vec == 4 && !c.args.evenCapGiven) ? StreamShape{bs, 0}
+	                       : stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
 	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, order, c.stream, nF, nC,
 	                       c.args.adjRle != 0, c.args.nontemporal != 0};   // default on (-1)
 	const float f16s = (float)(1 << c.args.f16Shift);
