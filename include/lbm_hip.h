@@ -248,7 +248,8 @@ int LbmHipBatchGetTuning(LbmHipBatch * b, LbmHipBatchTuning * out);
 int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
 /* Times `pairs` launches of every candidate shape on the active domains and
  * keeps the fastest, then `pairs` step pairs for groups of 0 (none), 64,
- * 128, 192, 256, 384 and 512 MiB with that shape; the lattices are restored
+ * 128, 192, 256, 384, 512, 768 and 1024 MiB with that shape, then the
+ * shapes again under the chosen groups; the lattices are restored
  * afterwards. */
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */

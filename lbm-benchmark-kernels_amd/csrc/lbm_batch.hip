@@ -1018,7 +1018,7 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 	// cache-sized groups with the chosen shapes: whole pairs, group by group
 	float bestPair = 1e30f;
 	best.groupMB = 0;
-	for (int mb : {0, 64, 128, 192, 256, 384, 512}) {
+	for (int mb : {0, 64, 128, 192, 256, 384, 512, 768, 1024}) {
 		if (!ok) break;
 		b->tune = best;
 		b->tune.groupMB = mb;

@@ -72,7 +72,7 @@ def test_batch_launch_shapes(lbm, prec):
             if shape == "auto":
                 t = b.autotune(2)
                 assert t.evenMs > 0 and t.oddMs > 0 and t.pairMs > 0 and 1 <= t.vec <= vmax
-                assert t.groupMB in (0, 64, 128, 192, 256, 384, 512)
+                assert t.groupMB in (0, 64, 128, 192, 256, 384, 512, 768, 1024)
             else:
                 b.set_tuning(*shape)
             b.step(20)
