@@ -47,11 +47,12 @@
  *                     starts at f = 0 in every storage type.
  *   -fp16-scale <2^k> power-of-two factor of the FP16 deviations
  *                     (default 4096)
- *   -statistics       list only: call the program's KernelStatistics during the
- *                     run as the reference's STATISTICS build does; with
+ *   -statistics       call the program's KernelStatistics during the run as
+ *                     the reference's STATISTICS build does; with
  *                     CaseData.VtkOutput set the program's VtkWrite is called
- *                     at the reference's points as well
- *                     (src/BenchKernelD3Q19ListAaPv.c:67-156)
+ *                     at the reference's points as well, with or without
+ *                     -statistics (src/BenchKernelD3Q19ListAaPv.c:67-156,
+ *                     src/BenchKernelD3Q19Aa.c:102-112, 296-332, 513-547)
  *   -shadow eager|lazy list only: allocate the host copy at Init (default) or
  *                     at the first LbmHipSyncShadow / download
  *   -odd-order band|group|plain

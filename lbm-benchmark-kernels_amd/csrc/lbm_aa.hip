@@ -24,6 +24,13 @@
 // lines (four, two or one).
 #include "lbm_list.h"      // storage codecs, elem_bytes, kW
 
+// The program's output functions (the reference's Vtk.o / Kernel.o, or
+// liblbmhost.so); weak, so the library loads without them.
+extern "C" {
+void VtkWrite(LatticeDesc * ld, KernelData * kd, CaseData * cd, int iteration) __attribute__((weak));
+void KernelStatistics(KernelData * kd, LatticeDesc * ld, CaseData * cd, int iteration) __attribute__((weak));
+}
+
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -56,6 +63,10 @@ struct AaData {
 	size_t nPdf = 0;
 	double * dMoments = nullptr;  // per-block pairs, then the total (allocated at first use)
 	uint32_t momentBlocks = 0;
+	// run-time output after an even step: GetNode serves the reference's
+	// view at even parity (pull view, force added) from the host copy
+	bool pullView = false;
+	double viewFt[19] = {};
 };
 
 static AaData * as_aa(KernelData * kd) { return reinterpret_cast<AaData *>(kd); }
@@ -312,9 +323,33 @@ static void aa_shadow_set(AaData * A, uint64_t i, int d, double v)
 
 static size_t aa_bytes(const AaData * A) { return elem_bytes(A->c.args.prec) * A->nPdf; }
 
+// Slot that direction d of cell (x, y, z) is pulled from after an even
+// step: inv(d) of the pull neighbour x - c_d (its periodic image, or the
+// ghost cell on an open face), or the cell's own slot d when that neighbour
+// is solid -- the source the odd step loads, and the slot the reference's
+// bounce-back fix-ups fill (src/BenchKernelD3Q19Aa.c:301-311).
+static uint64_t aa_pull_slot(const AaData * A, int x, int y, int z, int d)
+{
+	const AaGeom & g = A->g;
+	if (d == dC) return aa_slot(g, d, x, y, z);
+	const int id = inv_dir(d);
+	if (A->hInfo[((size_t)x * g.Y + y) * g.Z + z] & (1u << id)) return aa_slot(g, d, x, y, z);
+	int qx = x - cx(d), qy = y - cy(d), qz = z - cz(d);
+	aa_resolve(g, qx, qy, qz);
+	return aa_slot(g, id, qx, qy, qz);
+}
+
 static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
+	if (A->pullView) {
+		if (x < 0 || y < 0 || z < 0 || x >= A->g.X || y >= A->g.Y || z >= A->g.Z) {
+			fail("aa-soa: node (%d, %d, %d) outside the lattice.", x, y, z);
+			return;
+		}
+		for (int d = 0; d < 19; ++d) pdfs[d] = aa_shadow_get(A, aa_pull_slot(A, x, y, z, d), d) + A->viewFt[d];
+		return;
+	}
 	if (!aa_local_layout(A, x, y, z)) return;
 	for (int d = 0; d < 19; ++d) pdfs[d] = aa_shadow_get(A, aa_slot(A->g, d, x, y, z), d);
 }
@@ -322,6 +357,7 @@ static void aa_get_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 static void aa_set_node(KernelData * kd, int x, int y, int z, PdfT * pdfs)
 {
 	AaData * A = as_aa(kd);
+	if (A->pullView) { fail("aa-soa: SetNode during run-time output is not supported."); return; }
 	if (!aa_local_layout(A, x, y, z) || !shadow_writable(&A->c)) return;
 	for (int d = 0; d < 19; ++d) aa_shadow_set(A, aa_slot(A->g, d, x, y, z), d, pdfs[d]);
 	A->c.shadowDirty = 1;
@@ -379,8 +415,8 @@ __global__ __launch_bounds__(256) void k_aa_moments(const typename Codec::S * __
 	}
 }
 
-// One even + odd step pair.  ev: four events around the two launches
-// (per-launch timing) or null.
+// One even + odd step pair, or one of its halves.  ev: four events around
+// the two launches (per-launch timing) or null.
 struct AaLaunch {
 	dim3 grid, block;        // odd step: one z row segment per block
 	dim3 egrid, eblock;      // even step: blockDim.y rows of y per block
@@ -393,34 +429,41 @@ struct AaLaunch {
 };
 
 template <class Codec>
-static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, const TrtConsts & k0, const ForceTerms & f0,
-                      bool force, bool fl, bool fs, hipEvent_t * ev)
+static void aa_launch_even(const AaLaunch & L, typename Codec::S * pdf, Codec cod, const TrtC<typename Codec::R> & k,
+                           const ForceT<typename Codec::R> & ft, bool fl)
 {
-	using R = typename Codec::R;
-	const TrtC<R> k = trt_cast<R>(k0);
-	const ForceT<R> ft = force_cast<R>(f0);
 	constexpr int V = Codec::VMAX;
-	if (ev) (void)hipEventRecord(ev[0], L.stream);
-	bool vecDone = false;
 	if constexpr (V > 1) {
 		if (L.vec == V) {
-			vecDone = true;
 			if (fl && L.nt)  k_aa_even_v<Codec, V, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 			else if (fl)     k_aa_even_v<Codec, V, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 			else if (L.nt)   k_aa_even_v<Codec, V, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 			else             k_aa_even_v<Codec, V, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			return;
 		}
 	}
-	if (!vecDone) {
-		if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-		else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-		else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-		else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+}
+
+// doEven / doOdd: launch that half of the pair (all events of a pair are
+// recorded when both halves of it are launched, in one call or two).
+template <class Codec>
+static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, const TrtConsts & k0, const ForceTerms & f0,
+                      bool force, bool fl, bool fs, hipEvent_t * ev, bool doEven, bool doOdd)
+{
+	using R = typename Codec::R;
+	const TrtC<R> k = trt_cast<R>(k0);
+	const ForceT<R> ft = force_cast<R>(f0);
+	if (doEven) {
+		if (ev) (void)hipEventRecord(ev[0], L.stream);
+		aa_launch_even(L, pdf, cod, k, ft, fl);
+		if (ev) (void)hipEventRecord(ev[1], L.stream);
 	}
-	if (ev) {
-		(void)hipEventRecord(ev[1], L.stream);
-		(void)hipEventRecord(ev[2], L.stream);
-	}
+	if (!doOdd) return;
+	if (ev) (void)hipEventRecord(ev[2], L.stream);
 	// the odd step loads the force increment of the even step whenever the force is on
 	if (fs)          k_aa_odd<Codec, true, true><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 	else if (force)  k_aa_odd<Codec, true, false><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
@@ -451,17 +494,50 @@ static bool aa_upload(AaData * A)
 	return true;
 }
 
-static bool aa_run(KernelData * kd, const CaseData * cd)
+// Host copy of the current state for the output functions' GetNode.
+static bool aa_output_sync(AaData * A)
+{
+	if (A->c.shadowDirty || !A->c.shadowStale) return true;   // the host copy is current
+	return aa_download(&A->kd);
+}
+
+static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 {
 	AaData * A = as_aa(kd);
 	Common & c = A->c;
-	if (cd->VtkOutput) {
-		// the reference's aa-soa writes some snapshots between the even and
-		// the odd step, in the swapped layout (src/BenchKernelD3Q19Aa.c:319-331)
-		return fail("aa-soa: VTK output during the run is built for list-aa-pv-soa only.");
-	}
 	const int steps = cd->MaxIterations > 0 ? cd->MaxIterations + (cd->MaxIterations & 1) : 0;
 	if (!(cd->Omega > 0.0 && cd->Omega < 2.0)) return fail("omega must be in (0, 2), got %f.", cd->Omega);
+
+	// Run-time output at the points of the reference's VTK_OUTPUT / STATISTICS
+	// build (src/BenchKernelD3Q19Aa.c:102-112, 296-332, 513-547): before the
+	// loop; after each even step, once on the state before the bounce-back
+	// fix-ups (the local view of the swapped values) and once on the view at
+	// even parity (pull view, force added); after each odd step.  The reference
+	// tests (iter + 1 % VtkModulus) == 0 after the odd step, which holds only
+	// for VtkModulus 1 at iter 0; that is kept.
+	const bool vtk = ld && cd->VtkOutput;
+	const bool stats = ld && c.args.statistics;
+	if (vtk && !VtkWrite) return fail("VTK output requested but the program does not link VtkWrite.");
+	if (stats && !KernelStatistics) return fail("-statistics requested but the program does not link KernelStatistics.");
+	if (vtk && cd->VtkModulus <= 0) return fail("VtkModulus must be > 0.");
+	if (stats && cd->StatisticsModulus <= 0) return fail("StatisticsModulus must be > 0.");
+	CaseData * ccd = const_cast<CaseData *>(cd);
+	auto due = [&](int i) { return i % 10 == 0 || i % cd->StatisticsModulus == 0; };
+	if ((vtk || stats) && !A->hInfo) {
+		const size_t n = (size_t)A->g.X * A->g.Y * A->g.Z;
+		A->hInfo = (uint32_t *)malloc(sizeof(uint32_t) * n);
+		if (!A->hInfo) return fail("allocating %zu bytes failed.", sizeof(uint32_t) * n);
+		if (!hip_ok(hipMemcpy(A->hInfo, A->dInfo, sizeof(uint32_t) * n, hipMemcpyDeviceToHost), "cell info download")) {
+			free(A->hInfo);
+			A->hInfo = nullptr;
+			return false;
+		}
+	}
+	if (vtk || stats) {
+		if (!aa_output_sync(A)) return false;
+		if (vtk) VtkWrite(ld, kd, ccd, -1);
+		if (stats) KernelStatistics(kd, ld, ccd, 0);
+	}
 
 	if (!aa_upload(A)) return false;
 
@@ -494,21 +570,56 @@ static bool aa_run(KernelData * kd, const CaseData * cd)
 	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && wideDefault);
 	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
-	auto pair = [&](bool on, bool fl, bool fs, int it) {
+	auto half = [&](bool on, bool fl, bool fs, int it, bool doEven, bool doOdd) {
+		hipEvent_t * ev = perLaunch ? c.ev + 2 * it : nullptr;
 		switch (c.args.prec) {
-			case PREC_FP32S: aa_launch(L, (float *)A->dPdf, CodecF32{}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr); break;
+			case PREC_FP32S: aa_launch(L, (float *)A->dPdf, CodecF32{}, k, ft, on, fl, fs, ev, doEven, doOdd); break;
 			case PREC_FP16S:
-				aa_launch(L, (uint16_t *)A->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr);
+				aa_launch(L, (uint16_t *)A->dPdf, CodecF16{f16s, 1.0f / f16s}, k, ft, on, fl, fs, ev, doEven, doOdd);
 				break;
-			default: aa_launch(L, (double *)A->dPdf, CodecF64{}, k, ft, on, fl, fs, perLaunch ? c.ev + 2 * it : nullptr);
+			default: aa_launch(L, (double *)A->dPdf, CodecF64{}, k, ft, on, fl, fs, ev, doEven, doOdd);
 		}
+	};
+	// the host copy after a launch: the kernels so far, then a download
+	auto output_state = [&]() -> bool {
+		if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
+		c.shadowStale = 1;
+		return aa_download(kd);
 	};
 
 	(void)hipGetLastError();   // launch checks below must see only this loop's errors
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
+	bool segmentStart = true;  // the force of the previous step is in the stored values
+	bool synced = true;        // the host copy holds the state of the last launch
 	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
-		const bool last = it + 2 >= steps;
-		pair(force, force && it > 0, force && last, it);
+		const bool statsEven = stats && due(it), vtkEven = vtk && it % cd->VtkModulus == 0;
+		const bool vtkOdd = vtk && cd->VtkModulus == 1 && it == 0, statsOdd = stats && due(it + 1);
+		const bool mid = statsEven || vtkEven, end = vtkOdd || statsOdd;
+		const bool last = it + 2 >= steps || end;
+		synced = false;
+		if (!mid) {
+			half(force, force && !segmentStart, force && last, it, true, true);
+		} else {
+			half(force, force && !segmentStart, false, it, true, false);
+			if (!output_state()) return false;
+			c.iteration = it - 1;              // before the fix-ups: the swapped values in the local view
+			if (statsEven) KernelStatistics(kd, ld, ccd, it);
+			for (int d = 0; d < 19; ++d) A->viewFt[d] = force ? ft.f[d] : 0.0;
+			A->pullView = true;
+			c.iteration = it;
+			if (vtkEven) VtkWrite(ld, kd, ccd, it);
+			if (statsEven) KernelStatistics(kd, ld, ccd, it);
+			A->pullView = false;
+			half(force, false, force && last, it, false, true);
+		}
+		segmentStart = last;
+		if (end) {
+			if (!output_state()) return false;
+			synced = true;
+			c.iteration = it + 1;
+			if (vtkOdd) VtkWrite(ld, kd, ccd, it + 1);
+			if (statsOdd) KernelStatistics(kd, ld, ccd, it + 1);
+		}
 	}
 	if (!hip_ok(hipGetLastError(), "step kernel launch")) return false;
 	if (!hip_ok(hipEventRecord(c.evLoop[1], c.stream), "hipEventRecord")) return false;
@@ -519,19 +630,19 @@ static bool aa_run(KernelData * kd, const CaseData * cd)
 	kd->Duration = c.t.loopMs / 1e3;
 	if (steps > 0) c.iteration = steps - 1;
 
-	c.shadowStale = A->nFluid > 0 && steps > 0;
-	if (c.shadowDownload) {
+	if (!synced) c.shadowStale = A->nFluid > 0 && steps > 0;
+	if (c.shadowDownload || vtk) {
 		const double t0 = (double)clock() / CLOCKS_PER_SEC;
-		if (!aa_download(kd)) return false;
+		if (!aa_output_sync(A)) return false;
 		c.t.downloadMs = ((double)clock() / CLOCKS_PER_SEC - t0) * 1e3;
 	}
+	if (vtk) VtkWrite(ld, kd, ccd, cd->MaxIterations);
 	return true;
 }
 
 static void aa_kernel(LatticeDesc * ld, KernelData * kd, CaseData * cd)
 {
-	(void)ld;
-	(void)aa_run(kd, cd);
+	(void)aa_run(kd, cd, ld);
 }
 
 
@@ -543,7 +654,7 @@ Common * aa_common(KernelData * kd)
 bool aa_steps(KernelData * kd, const CaseData * cd)
 {
 	if (!aa_common(kd)) return fail("not an aa-soa kernel.");
-	return aa_run(kd, cd);
+	return aa_run(kd, cd, nullptr);
 }
 
 bool aa_download_shadow(KernelData * kd)

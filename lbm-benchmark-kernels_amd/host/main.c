@@ -182,14 +182,10 @@ int main(int argc, char * argv[])
 	}
 	printf("#\n# - kernel:            %s\n#\n", kf->Name);
 
-	/* -statistics: the list kernel writes KernelStatistics during the run
-	 * (the reference's STATISTICS build) when given the kernel argument */
+	/* -statistics: the kernels write KernelStatistics during the run (the
+	 * reference's STATISTICS build) when given the kernel argument */
 	char ** kargs = NULL;
 	if (statistics) {
-		if (strcasecmp(kf->Name, "list-aa-pv-soa") != 0) {
-			printf("ERROR: -statistics during the run is built for list-aa-pv-soa only.\n");
-			return 1;
-		}
 		kargs = (char **)malloc(sizeof(char *) * (size_t)(p.nKernelArgs + 1));
 		if (!kargs) { printf("ERROR: allocating kernel arguments failed.\n"); return 1; }
 		for (int j = 0; j < p.nKernelArgs; ++j) kargs[j] = p.KernelArgs[j];

@@ -141,7 +141,7 @@ def test_kernel_argument_errors(lbm, args, msg):
 
 
 @pytest.mark.parametrize("args", [["-pad", "auto"], ["-adj", "full"], ["-shadow", "lazy"],
-                                  ["-odd-order", "band"], ["-statistics"]])
+                                  ["-odd-order", "band"]])
 def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):

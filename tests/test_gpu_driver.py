@@ -65,6 +65,13 @@ def test_verify_with_run_time_output(tmp_path):
     rows = [ln.split() for ln in open(tmp_path / "ux-progress.dat") if not ln.startswith("#")]
     iters = [int(x[0]) for x in rows]
     assert iters[0] == 0 and iters[-1] == 1000 and 500 in iters and 990 in iters
-    r = subprocess.run([DRIVER, "-verify", "-kernel", "aa-soa", "-statistics", "100"], capture_output=True,
-                       text=True, cwd=tmp_path, timeout=120)
-    assert r.returncode != 0 and "list-aa-pv-soa only" in r.stdout
+    aa = tmp_path / "aa"
+    aa.mkdir()
+    r = subprocess.run([DRIVER, "-verify", "-kernel", "aa-soa", "-vtk", "500", "-statistics", "100"],
+                       capture_output=True, text=True, cwd=aa, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "# VERIFICATION: deviation from analytical solution: 4.561374e-02" in r.stdout
+    for it in ("-001", "0000", "0500", "1000"):
+        assert (aa / "vtk" / f"file-{it}.vtk").exists(), it
+    iters = [int(ln.split()[0]) for ln in open(aa / "ux-progress.dat") if not ln.startswith("#")]
+    assert iters[0] == 0 and 500 in iters and 990 in iters

@@ -78,10 +78,47 @@ def test_run_time_output_matches_replay(lbm, oracle, tmp_path):
         assert got[name] == want[name], name
 
 
-def test_aa_rejects_run_time_vtk(lbm):
-    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
-    with lbm.Kernel("aa-soa", geo) as k:
-        cd = lbm.case_data(iterations=2)
-        cd.VtkOutput = 1
-        with pytest.raises(RuntimeError, match="list-aa-pv-soa only"):
+@pytest.mark.parametrize("geom,dims,per,steps,vmod,smod", [
+    ("channel", (16, 12, 10), (1, 0, 0), 24, 4, 6),
+    ("blocks-2", (14, 12, 10), (0, 0, 0), 12, 3, 4),      # open x faces: the ghost layer in the views
+    ("box", (10, 8, 8), (1, 1, 0), 6, 1, 3),              # VtkModulus 1: the reference's odd-step snapshot at 1
+])
+def test_aa_run_time_output_matches_reference(lbm, tmp_path, geom, dims, per, steps, vmod, smod):
+    """aa-soa writes the files of the reference's VTK_OUTPUT / STATISTICS build
+    (VTK snapshots, ux-progress.dat, density-ux.dat) byte for byte: the
+    reference's own aa-soa kernel, compiled with both into
+    oracle/_ref/libref_io.so, runs the same case on the CPU."""
+    from lbm_checkers import Reference
+    if not Reference.available("io"):
+        pytest.skip("oracle/_ref/libref_io.so not built (needs the reference sources)")
+    ref = Reference("io")
+    geo = lbm.Geometry.from_string(geom, dims, per)
+    m = geo.mask.reshape(-1)
+    omega, F = 1.1, 2e-5
+    print(f"aa run-time output {geom} {dims} steps={steps} vtk every {vmod} statistics every {smod}", flush=True)
+
+    def reference_run():
+        ref.lib.ref_set_output(1, vmod, smod)
+        try:
+            want_nodes, _ = ref.aa_run(m, geo.dims, geo.periodic, steps, omega, F, init="weights")
+        finally:
+            ref.lib.ref_set_output(0, 1, 1)
+        return want_nodes
+    want_nodes = _in_dir(str(tmp_path / "ref"), reference_run)
+
+    def gpu():
+        with lbm.Kernel("aa-soa", geo, ["-statistics"]) as k:
+            k.set_initial_weights()
+            cd = lbm.case_data(omega=omega, xforce=F, iterations=steps)
+            cd.VtkOutput, cd.VtkModulus, cd.StatisticsModulus = 1, vmod, smod
             k.run(cd)
+            return k.dump()
+    got_nodes = _in_dir(str(tmp_path / "gpu"), gpu)
+    assert np.array_equal(got_nodes, want_nodes)
+    got, want = _files(str(tmp_path / "gpu")), _files(str(tmp_path / "ref"))
+    assert sorted(got) == sorted(want)
+    assert "vtk/file--001.vtk" in got and f"vtk/file-{steps:04d}.vtk" in got
+    if vmod == 1:
+        assert "vtk/file-0001.vtk" in got
+    for name in want:
+        assert got[name] == want[name], name
