@@ -22,6 +22,11 @@
 // and zStride a multiple of zOff, so that interior
 // z rows start on a 128-B line and a wavefront's 64 lanes cover whole
 // lines (four, two or one).
+//
+// Storage: FP64 values, or with -prec fp32s / fp16s the deviations f - w_d
+// through the list kernel's codecs (the narrow even step takes 2 / 4 cells
+// of a z row per thread).  Run-time VTK / statistics output follows the
+// reference's call points, including the views after an even step.
 #include "lbm_list.h"      // storage codecs, elem_bytes, kW
 
 // The program's output functions (the reference's Vtk.o / Kernel.o, or
