@@ -122,3 +122,39 @@ def test_aa_run_time_output_matches_reference(lbm, tmp_path, geom, dims, per, st
         assert "vtk/file-0001.vtk" in got
     for name in want:
         assert got[name] == want[name], name
+
+
+def _vtk_fields(raw, n):
+    head = raw.index(b"LOOKUP_TABLE default\n") + len(b"LOOKUP_TABLE default\n")
+    p = raw.index(b"LOOKUP_TABLE default\n", head + n) + len(b"LOOKUP_TABLE default\n")
+    rho = np.frombuffer(raw[p:p + 8 * n], dtype=">f8")
+    q = raw.index(b"VECTORS VelocityVectors double\n", p + 8 * n) + len(b"VECTORS VelocityVectors double\n")
+    return rho, np.frombuffer(raw[q:q + 24 * n], dtype=">f8").reshape(n, 3)
+
+
+@pytest.mark.parametrize("name", ["aa-soa", "list-aa-pv-soa"])
+def test_shifted_storage_run_time_output(lbm, tmp_path, name):
+    """Run-time output with shifted FP32 storage writes the same files as
+    FP64, with density and velocity within FP32 accuracy of the FP64 run."""
+    geo = lbm.Geometry.from_string("channel", (16, 12, 10), (1, 0, 0))
+    n = int(np.prod(geo.dims))
+    steps, vmod = 12, 4
+
+    def run(prec):
+        def go():
+            with lbm.Kernel(name, geo, ["-statistics", "-prec", prec]) as k:
+                k.set_initial_weights()
+                cd = lbm.case_data(omega=1.1, xforce=2e-5, iterations=steps)
+                cd.VtkOutput, cd.VtkModulus, cd.StatisticsModulus = 1, vmod, 4
+                k.run(cd)
+        _in_dir(str(tmp_path / prec), go)
+        return _files(str(tmp_path / prec))
+    f64, f32 = run("fp64"), run("fp32s")
+    assert sorted(f64) == sorted(f32) and any(k.startswith("vtk/") for k in f64)
+    for name_ in f64:
+        if not name_.startswith("vtk/"):
+            continue
+        r64, u64 = _vtk_fields(f64[name_], n)
+        r32, u32 = _vtk_fields(f32[name_], n)
+        assert np.abs(r32 - r64).max() < 1e-6, name_
+        assert np.abs(u32 - u64).max() < 1e-7, name_
