@@ -310,13 +310,14 @@ def aa_mode(args, lbm):
             k.run(lbm.case_data(omega=1.0, xforce=1e-5, iterations=steps))
             t = k.timings()
         nf = geo.n_fluid
-        b = 2 * 19 * {"fp64": 8, "fp32s": 4, "fp16s": 2}[prec] + 4   # PDFs plus the cell's info word
+        pdf_b = 2 * 19 * {"fp64": 8, "fp32s": 4, "fp16s": 2}[prec]
+        be, bo = pdf_b + 1 / 8, pdf_b + 4   # PDFs plus the fluid bit (even) or the cell's info word (odd)
         key = f"{edge}^3" + ("" if prec == "fp64" else f" {prec}")
         out[key] = {"n_fluid": nf, "steps": steps, "mflups": round(nf * steps / (loop_ms / 1e3) / 1e6, 1),
                     "mflups_with_launch_events": round(nf * steps / (t.loopMs / 1e3) / 1e6, 1),
                     "even_ms": round(t.evenMsAvg, 5), "odd_ms": round(t.oddMsAvg, 5),
-                    "even_gbs": round(b * nf / (t.evenMsAvg / 1e3) / 1e9, 1),
-                    "odd_gbs": round(b * nf / (t.oddMsAvg / 1e3) / 1e9, 1)}
+                    "even_gbs": round(be * nf / (t.evenMsAvg / 1e3) / 1e9, 1),
+                    "odd_gbs": round(bo * nf / (t.oddMsAvg / 1e3) / 1e9, 1)}
         _log(f"aa-soa {key}: {out[key]}")
     return _common("MFLUP/s (D3Q19 TRT aa-soa FP64, 256^3 duct channel)", out["256^3"]["mflups"], "MFLUP/s", 200,
                    "aa-soa, channel (x periodic, walls y/z), omega 1.0, Lambda 3/16, F 1e-5, FP64, f = w at start",

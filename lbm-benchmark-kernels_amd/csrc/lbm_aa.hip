@@ -52,6 +52,7 @@ struct AaGeom {
 	int gY;
 	int zStride;
 	int zOff;                // 128 B of stored values
+	int zWords;              // 64-bit words of the fluid bit mask per z row
 	uint64_t plane;
 };
 
@@ -62,6 +63,7 @@ struct AaData {
 	int nFluid = 0;
 	int nBounce = 0;
 	uint32_t * dInfo = nullptr;   // [x][y][z], z fastest
+	unsigned long long * dFluid = nullptr;   // fluid bit per cell, [x][y][z / 64] words (even step)
 	void * dPdf = nullptr;        // nPdf values of the storage type (-prec)
 	void * hPdf = nullptr;
 	uint32_t * hInfo = nullptr;
@@ -128,6 +130,27 @@ __global__ __launch_bounds__(256) void k_aa_info(const int * __restrict__ mask, 
 	if (__lane_id() == 0 && bounces) atomicAdd(nBounce, (uint32_t)bounces);
 }
 
+// Fluid bit of every cell, 64 cells of a z row per word: what the even step
+// reads instead of the 4-byte info word.
+__global__ __launch_bounds__(256) void k_aa_fluid_bits(const uint32_t * __restrict__ info, AaGeom g,
+                                                       unsigned long long * __restrict__ bits)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= (uint64_t)g.X * g.Y * g.zWords) return;
+	const uint64_t row = i / (uint64_t)g.zWords;
+	const int z0 = (int)(i % (uint64_t)g.zWords) * 64;
+	unsigned long long w = 0;
+	for (int j = 0; j < 64 && z0 + j < g.Z; ++j)
+		if (info[row * (uint64_t)g.Z + z0 + j] & kInfoFluid) w |= 1ull << j;
+	bits[i] = w;
+}
+
+__device__ __forceinline__ unsigned long long aa_fluid_word(const unsigned long long * bits, const AaGeom & g, int x, int y,
+                                                            int z)
+{
+	return bits[((size_t)x * g.Y + y) * g.zWords + (z >> 6)];
+}
+
 // Cell of this thread: z along threadIdx.x, blockDim.y rows of y per block.
 __device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int & z)
 {
@@ -140,25 +163,34 @@ __device__ __forceinline__ bool aa_cell(const AaGeom & g, int & x, int & y, int 
 // Even step: local load, collision, store of direction d into slot inv(d)
 // of the same cell (src/BenchKernelD3Q19Aa.c:146-290).
 // NT: non-temporal loads and stores (each value is touched once per step).
+// Solid cells of the box load their slots and store them back unchanged, so
+// that every wavefront writes whole lines (a non-temporal store of a partly
+// written line costs far more than the bytes saved; no fluid cell ever reads
+// a solid cell's slots).
 template <class Codec, bool FORCE_LOAD, bool NT>
-__global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
-                                                 AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+__global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec cod,
+                                                 const unsigned long long * __restrict__ fluidBits, AaGeom g,
+                                                 TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
 	using R = typename Codec::R;
 	int x, y, z;
 	if (!aa_cell(g, x, y, z)) return;
-	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
-	if (!(w & kInfoFluid)) return;
+	const bool fluid = (aa_fluid_word(fluidBits, g, x, y, z) >> (z & 63)) & 1ull;
 	const uint64_t base = aa_slot(g, 0, x, y, z);
 	R f[19];
 #pragma unroll
-	for (int d = 0; d < 19; ++d) {
-		f[d] = cod.template load<NT>(&pdf[base + (uint64_t)d * g.plane]);
-		if (FORCE_LOAD) f[d] = f[d] + ft.f[d];
-	}
-	trt_collide(f, k);
+	for (int d = 0; d < 19; ++d) f[d] = cod.template load<NT>(&pdf[base + (uint64_t)d * g.plane]);
+	if (fluid) {
+		if (FORCE_LOAD) {
 #pragma unroll
-	for (int d = 0; d < 19; ++d) cod.template store<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], f[d]);
+			for (int d = 0; d < 19; ++d) f[d] = f[d] + ft.f[d];
+		}
+		trt_collide(f, k);
+	}
+	// slot inv(d) takes f_d of a fluid cell, its own value for a solid one
+#pragma unroll
+	for (int d = 0; d < 19; ++d)
+		cod.template store<NT>(&pdf[base + (uint64_t)inv_dir(d) * g.plane], fluid ? f[d] : f[inv_dir(d)]);
 }
 
 // Even step over V consecutive cells z0 .. z0+V-1 of a z row, for the
@@ -170,8 +202,9 @@ __global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec 
 // the cell's own value inv(d) (a decode/encode round trip is exact).  Each
 // fluid cell's arithmetic is that of k_aa_even.
 template <class Codec, int V, bool FORCE_LOAD, bool NT>
-__global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
-                                                   AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+__global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Codec cod,
+                                                   const unsigned long long * __restrict__ fluidBits, AaGeom g,
+                                                   TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
 {
 	using R = typename Codec::R;
 	using VT = VecS<typename Codec::S, V>;
@@ -179,12 +212,13 @@ __global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Code
 	const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
 	const int x = (int)blockIdx.z;
 	if (z0 >= g.Z || y >= g.Y) return;
-	const size_t i0 = ((size_t)x * g.Y + y) * g.Z + z0;
+	// z0 .. z0+V-1 lie in one word of the mask (V divides 64)
+	const unsigned long long word = aa_fluid_word(fluidBits, g, x, y, z0) >> (z0 & 63);
 	bool fluid[V];
 	bool any = false;
 #pragma unroll
 	for (int j = 0; j < V; ++j) {
-		fluid[j] = z0 + j < g.Z && (info[i0 + j] & kInfoFluid);
+		fluid[j] = z0 + j < g.Z && ((word >> j) & 1ull);
 		any = any || fluid[j];
 	}
 	if (!any) return;
@@ -428,7 +462,8 @@ struct AaLaunch {
 	size_t elds;
 	int vec;                 // even step: cells per thread (1, or VMAX of the codec)
 	hipStream_t stream;
-	const uint32_t * info;
+	const uint32_t * info;               // odd step
+	const unsigned long long * fluid;    // even step
 	AaGeom g;
 	bool nt;
 };
@@ -440,17 +475,17 @@ static void aa_launch_even(const AaLaunch & L, typename Codec::S * pdf, Codec co
 	constexpr int V = Codec::VMAX;
 	if constexpr (V > 1) {
 		if (L.vec == V) {
-			if (fl && L.nt)  k_aa_even_v<Codec, V, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-			else if (fl)     k_aa_even_v<Codec, V, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-			else if (L.nt)   k_aa_even_v<Codec, V, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-			else             k_aa_even_v<Codec, V, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			if (fl && L.nt)  k_aa_even_v<Codec, V, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+			else if (fl)     k_aa_even_v<Codec, V, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+			else if (L.nt)   k_aa_even_v<Codec, V, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+			else             k_aa_even_v<Codec, V, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
 			return;
 		}
 	}
-	if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
-	else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+	if (fl && L.nt)  k_aa_even<Codec, true, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+	else if (fl)     k_aa_even<Codec, true, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+	else if (L.nt)   k_aa_even<Codec, false, true><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
+	else             k_aa_even<Codec, false, false><<<L.egrid, L.eblock, L.elds, L.stream>>>(pdf, cod, L.fluid, L.g, k, ft);
 }
 
 // doEven / doOdd: launch that half of the pair (all events of a pair are
@@ -562,18 +597,18 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const int zq = (g.Z + vec - 1) / vec;
 	const int ebs = zq >= 256 ? 256 : ((zq + 63) / 64) * 64;
 	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
-	// Defaults measured at 64^3 and 256^3 (duct channel): FP64 runs best with
-	// plain accesses and the occupancy cap; the narrow types on lattices larger
-	// than the 256 MiB last-level cache with non-temporal accesses and full
-	// occupancy (+8 %), on smaller ones like FP64 (non-temporal accesses would
-	// push the cache-resident lattice out).
-	const bool wideDefault = c.args.prec != PREC_FP64 && aa_bytes(A) > (size_t)256 << 20;
-	const StreamShape es = (wideDefault && !c.args.evenCapGiven)
+	// Defaults measured at 64^3 and 256^3 (duct channel): on lattices larger
+	// than the 256 MiB last-level cache non-temporal accesses (FP64 with the
+	// occupancy cap, the narrow types at full occupancy), on smaller ones plain
+	// accesses with the cap (non-temporal accesses would push the
+	// cache-resident lattice out).
+	const bool big = aa_bytes(A) > (size_t)256 << 20;
+	const StreamShape es = (big && c.args.prec != PREC_FP64 && !c.args.evenCapGiven)
 	                       ? StreamShape{ebs, 0} : stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
 	const unsigned rows = (unsigned)(es.block / ebs);
 	const dim3 eblock((unsigned)ebs, rows), egrid(egx, ((unsigned)g.Y + rows - 1) / rows, grid.z);
-	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && wideDefault);
-	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, g, nt};
+	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && big);
+	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, A->dFluid, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto half = [&](bool on, bool fl, bool fs, int it, bool doEven, bool doOdd) {
 		hipEvent_t * ev = perLaunch ? c.ev + 2 * it : nullptr;
@@ -711,6 +746,7 @@ static void aa_release(AaData * A)
 	if (!A) return;
 	(void)hipFree(A->dMoments);
 	(void)hipFree(A->dInfo);
+	(void)hipFree(A->dFluid);
 	(void)hipFree(A->dPdf);
 	host_free(A->hPdf);
 	free(A->hInfo);
@@ -739,6 +775,7 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	g.gY = g.Y + 2;
 	g.zOff = (int)(128 / elem_bytes(A->c.args.prec));
 	g.zStride = ((g.zOff + g.Z + 1 + g.zOff - 1) / g.zOff) * g.zOff;
+	g.zWords = (g.Z + 63) / 64;
 	g.plane = (uint64_t)(g.X + 2) * g.gY * g.zStride;
 	A->nPdf = (size_t)19 * g.plane;
 	A->nFluid = ld->nFluid;
@@ -760,11 +797,14 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		if (!hip_ok(hipMalloc((void **)&dMask, sizeof(int) * nTotal), "allocating geometry mask")) break;
 		if (!hip_ok(hipMalloc((void **)&dCount, sizeof(uint32_t)), "allocating counter")) break;
 		if (!hip_ok(hipMalloc((void **)&A->dInfo, sizeof(uint32_t) * nTotal), "allocating cell info")) break;
+		const uint64_t nWords = (uint64_t)g.X * g.Y * g.zWords;
+		if (!hip_ok(hipMalloc((void **)&A->dFluid, sizeof(unsigned long long) * nWords), "allocating fluid mask")) break;
 		if (!hip_ok(hipMalloc((void **)&A->dPdf, aa_bytes(A)), "allocating device lattice")) break;
 		if (!hip_ok(hipMemcpyAsync(dMask, ld->Lattice, sizeof(int) * nTotal, hipMemcpyHostToDevice, s), "mask upload")) break;
 		if (!hip_ok(hipMemsetAsync(dCount, 0, sizeof(uint32_t), s), "hipMemset")) break;
 		(void)hipGetLastError();
 		k_aa_info<<<(unsigned)((nTotal + 255) / 256), 256, 0, s>>>(dMask, g, A->dInfo, dCount);
+		k_aa_fluid_bits<<<(unsigned)((nWords + 255) / 256), 256, 0, s>>>(A->dInfo, g, A->dFluid);
 		if (!hip_ok(hipGetLastError(), "cell info kernel")) break;
 		if (!hip_ok(hipMemcpyAsync(&hCount, dCount, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count download")) break;
 		A->hPdf = host_alloc(aa_bytes(A));
