@@ -68,7 +68,9 @@
  *                     nodes (aa-soa: consecutive z cells) per thread with
  *                     8-byte accesses (auto, default; the list kernel only
  *                     when the padded node count allows), or one per
- *                     thread (1)
+ *                     thread (1); at fp32s aa-soa's odd step takes the
+ *                     same pairs of z cells, with one access per direction
+ *                     for a pair of interior fluid cells
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H

@@ -249,7 +249,8 @@ __global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Code
 	}
 }
 
-// Odd step: direction d arrives from the pull neighbour x - c_d, slot inv(d)
+// Odd step of one fluid cell x with info word w: direction d arrives from
+// the pull neighbour x - c_d, slot inv(d)
 // (src/BenchKernelD3Q19Aa.c:392) -- or from the cell's own slot d when that
 // neighbour is solid -- and leaves to the push neighbour x + c_d, slot d
 // (:435-487) -- or to the cell's own slot inv(d) when the target is solid.
@@ -261,14 +262,11 @@ __global__ __launch_bounds__(512) void k_aa_even_v(typename Codec::S * pdf, Code
 // node view the reference forces after the step, so it gets the increment
 // here too.
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE>
-__global__ __launch_bounds__(512) void k_aa_odd(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
-                                                AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+__device__ __forceinline__ void aa_odd_cell(typename Codec::S * pdf, Codec cod, const AaGeom & g,
+                                            const TrtC<typename Codec::R> & k, const ForceT<typename Codec::R> & ft,
+                                            int x, int y, int z, uint32_t w)
 {
 	using R = typename Codec::R;
-	int x, y, z;
-	if (!aa_cell(g, x, y, z)) return;
-	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
-	if (!(w & kInfoFluid)) return;
 	const bool edge = (w & kInfoBoundary) != 0;
 	const uint64_t self = aa_slot(g, 0, x, y, z);
 
@@ -335,6 +333,99 @@ __global__ __launch_bounds__(512) void k_aa_odd(typename Codec::S * pdf, Codec c
 				cod.template store<false>(&pdf[own], cod.template load<false>(&pdf[own]) + ft.f[d]);
 			}
 		}
+	}
+}
+
+template <class Codec, bool FORCE_LOAD, bool FORCE_STORE>
+__global__ __launch_bounds__(512) void k_aa_odd(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
+                                                AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+{
+	int x, y, z;
+	if (!aa_cell(g, x, y, z)) return;
+	const uint32_t w = info[((size_t)x * g.Y + y) * g.Z + z];
+	if (w & kInfoFluid) aa_odd_cell<Codec, FORCE_LOAD, FORCE_STORE>(pdf, cod, g, k, ft, x, y, z, w);
+}
+
+// V stored values of consecutive z cells as one access.  The pulls and
+// pushes of the odd step are offset by one cell in z, so the address is
+// aligned to one stored value only; the device's global memory
+// instructions accept that.
+template <class S, int V>
+__device__ __forceinline__ VecS<S, V> ld_run(const S * p)
+{
+	VecS<S, V> t;
+	__builtin_memcpy(&t, p, sizeof(t));
+	return t;
+}
+
+template <class S, int V>
+__device__ __forceinline__ void st_run(S * p, const VecS<S, V> & t)
+{
+	__builtin_memcpy(p, &t, sizeof(t));
+}
+
+// Odd step over groups of V consecutive z cells (FP32 pairs, -vec auto): a
+// group of interior cells (fluid, no solid neighbour, not on the
+// box boundary) moves each direction with one access of V values for the
+// pull and one for the push; any other group goes through aa_odd_cell cell
+// by cell.  Each cell's arithmetic is that of k_aa_odd.
+template <class Codec, int V, bool FORCE_LOAD, bool FORCE_STORE>
+__global__ __launch_bounds__(512) void k_aa_odd_v(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
+                                                  AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+{
+	using R = typename Codec::R;
+	using S = typename Codec::S;
+	const int z0 = (int)(blockIdx.x * blockDim.x + threadIdx.x) * V;
+	const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+	const int x = (int)blockIdx.z;
+	if (z0 >= g.Z || y >= g.Y) return;
+	const size_t i0 = ((size_t)x * g.Y + y) * g.Z + z0;
+	uint32_t w[V];
+	bool interior = true;
+#pragma unroll
+	for (int j = 0; j < V; ++j) {
+		w[j] = z0 + j < g.Z ? info[i0 + j] : 0u;
+		interior = interior && w[j] == kInfoFluid;
+	}
+	if (!interior) {
+#pragma unroll
+		for (int j = 0; j < V; ++j)
+			if (w[j] & kInfoFluid) aa_odd_cell<Codec, FORCE_LOAD, FORCE_STORE>(pdf, cod, g, k, ft, x, y, z0 + j, w[j]);
+		return;
+	}
+	const uint64_t self = aa_slot(g, 0, x, y, z0);
+	R f[V][19];
+#pragma unroll
+	for (int d = 0; d < 18; ++d) {
+		const VecS<S, V> v = ld_run<S, V>(&pdf[self + (uint64_t)inv_dir(d) * g.plane - (int64_t)cx(d) * g.gY * g.zStride -
+		                                      (int64_t)cy(d) * g.zStride - (int64_t)cz(d)]);
+#pragma unroll
+		for (int j = 0; j < V; ++j) f[j][d] = cod.dec(v[j]);
+	}
+	const uint64_t cSlot = self + (uint64_t)dC * g.plane;
+	{
+		const VecS<S, V> v = ld_run<S, V>(&pdf[cSlot]);
+#pragma unroll
+		for (int j = 0; j < V; ++j) f[j][dC] = cod.dec(v[j]);
+	}
+#pragma unroll
+	for (int j = 0; j < V; ++j) {
+		if (FORCE_LOAD) {
+#pragma unroll
+			for (int d = 0; d < 19; ++d) f[j][d] = f[j][d] + ft.f[d];
+		}
+		trt_collide(f[j], k);
+	}
+#pragma unroll
+	for (int d = 0; d < 19; ++d) {
+		// interior cells: every push target lies in the box, so the stored
+		// value takes this step's increment of its own direction
+		VecS<S, V> v;
+#pragma unroll
+		for (int j = 0; j < V; ++j) v[j] = cod.enc(FORCE_STORE ? f[j][d] + ft.f[d] : f[j][d]);
+		if (d == dC) st_run<S, V>(&pdf[cSlot], v);
+		else st_run<S, V>(&pdf[self + (uint64_t)d * g.plane + (int64_t)cx(d) * g.gY * g.zStride +
+		                       (int64_t)cy(d) * g.zStride + (int64_t)cz(d)], v);
 	}
 }
 
@@ -458,9 +549,10 @@ __global__ __launch_bounds__(256) void k_aa_moments(const typename Codec::S * __
 // the two launches (per-launch timing) or null.
 struct AaLaunch {
 	dim3 grid, block;        // odd step: one z row segment per block
+	dim3 vgrid, vblock;      // odd step over groups of vec cells (FP32)
 	dim3 egrid, eblock;      // even step: blockDim.y rows of y per block
 	size_t elds;
-	int vec;                 // even step: cells per thread (1, or VMAX of the codec)
+	int vec;                 // cells per thread in both steps (1, or VMAX of the codec)
 	hipStream_t stream;
 	const uint32_t * info;               // odd step
 	const unsigned long long * fluid;    // even step
@@ -505,6 +597,18 @@ static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, co
 	if (!doOdd) return;
 	if (ev) (void)hipEventRecord(ev[2], L.stream);
 	// the odd step loads the force increment of the even step whenever the force is on
+	// groups of cells only for 4-byte values: FP16 groups (8-byte accesses at
+	// 2-byte alignment) measured 1.5x slower than one cell per thread
+	constexpr int V = Codec::VMAX;
+	if constexpr (V > 1 && sizeof(typename Codec::S) == 4) {
+		if (L.vec == V) {
+			if (fs)          k_aa_odd_v<Codec, V, true, true><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			else if (force)  k_aa_odd_v<Codec, V, true, false><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			else             k_aa_odd_v<Codec, V, false, false><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
+			if (ev) (void)hipEventRecord(ev[3], L.stream);
+			return;
+		}
+	}
 	if (fs)          k_aa_odd<Codec, true, true><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 	else if (force)  k_aa_odd<Codec, true, false><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 	else             k_aa_odd<Codec, false, false><<<L.grid, L.block, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
@@ -590,9 +694,9 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const AaGeom g = A->g;
 	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
-	// even step: V cells per thread for the narrow types (-vec auto), blocks
-	// of 512 threads over several y rows at one block per CU, as the list
-	// kernel's even step (stream_shape)
+	// V cells per thread for the narrow types (-vec auto; the odd step at
+	// FP32 only); even step blocks of 512 threads over several y rows at
+	// one block per CU, as the list kernel's even step (stream_shape)
 	const int vec = c.args.evenVec == 0 ? 1 : c.args.prec == PREC_FP32S ? 2 : c.args.prec == PREC_FP16S ? 4 : 1;
 	const int zq = (g.Z + vec - 1) / vec;
 	const int ebs = zq >= 256 ? 256 : ((zq + 63) / 64) * 64;
@@ -608,7 +712,10 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const unsigned rows = (unsigned)(es.block / ebs);
 	const dim3 eblock((unsigned)ebs, rows), egrid(egx, ((unsigned)g.Y + rows - 1) / rows, grid.z);
 	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && big);
-	const AaLaunch L = {grid, block, egrid, eblock, es.lds, vec, c.stream, A->dInfo, A->dFluid, g, nt};
+	// odd step over groups of vec cells: blocks of 256 threads over 256 / ebs rows of y
+	const unsigned vrows = (unsigned)(256 / ebs);
+	const dim3 vblock((unsigned)ebs, vrows), vgrid(egx, ((unsigned)g.Y + vrows - 1) / vrows, (unsigned)g.X);
+	const AaLaunch L = {grid, block, vgrid, vblock, egrid, eblock, es.lds, vec, c.stream, A->dInfo, A->dFluid, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto half = [&](bool on, bool fl, bool fs, int it, bool doEven, bool doOdd) {
 		hipEvent_t * ev = perLaunch ? c.ev + 2 * it : nullptr;
