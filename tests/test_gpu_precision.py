@@ -248,6 +248,22 @@ def test_aa_shifted_open_faces_ragged_rows(lbm, oracle, prec):
             assert np.array_equal(got, want), (dims, extra)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_aa_shifted_rows_over_several_blocks(lbm, oracle, prec):
+    """z rows of 1100 cells with scattered solid cells, z periodic: the narrow
+    types' groups of z cells (FP32 pairs in both steps, FP16 groups of 4 in
+    the even step) span several blocks per row."""
+    dims, per = (5, 6, 1100), (0, 0, 1)
+    rng = np.random.default_rng(5)
+    mask = (rng.random(int(np.prod(dims))) > 0.05).astype(np.int32)
+    geo = lbm.Geometry.from_mask(mask, dims, per)
+    want = oracle.aa_run_shifted(mask, dims, per, [12, 8], 1.3, 2e-5, prec, magic=LAM)
+    for extra in ([], ["-vec", "1"]):
+        print(f"aa rows {dims} -prec {prec} {extra}", flush=True)
+        got = _aa_gpu(lbm, geo, [12, 8], 1.3, 2e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
+        assert np.array_equal(got, want), extra
+
+
 @pytest.mark.parametrize("prec", ["fp64"] + PRECS)
 def test_aa_moments_match_node_sums(lbm, prec):
     """aa-soa's FP64 device sums against sums of its node views, and
