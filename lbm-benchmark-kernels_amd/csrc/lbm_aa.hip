@@ -645,6 +645,27 @@ static bool aa_output_sync(AaData * A)
 	return aa_download(&A->kd);
 }
 
+// Threads along z of a block over rows of n cells (or cell groups), at
+// most 256: the odd step splits a row into equal parts rounded up to a
+// wavefront (384 cells: 2 x 192); the even step, whose blocks take
+// 512 / threads rows, takes the power of two from 64 to 256 that leaves
+// the fewest idle threads in a row's last block (384 cells: 3 x 128).
+static int odd_row_threads(int n)
+{
+	const int parts = (n + 255) / 256;
+	return ((n + parts - 1) / parts + 63) / 64 * 64;
+}
+
+static int even_row_threads(int n)
+{
+	int best = 256, idle = 1 << 30;
+	for (int b = 256; b >= 64; b /= 2) {
+		const int i = (n + b - 1) / b * b - n;
+		if (i < idle) { best = b; idle = i; }
+	}
+	return best;
+}
+
 static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 {
 	AaData * A = as_aa(kd);
@@ -692,14 +713,14 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	if (perLaunch && !common_events(&c, steps)) return false;
 
 	const AaGeom g = A->g;
-	const int bs = g.Z >= 256 ? 256 : ((g.Z + 63) / 64) * 64;
+	const int bs = odd_row_threads(g.Z);
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
 	// V cells per thread for the narrow types (-vec auto; the odd step at
 	// FP32 only); even step blocks of 512 threads over several y rows at
 	// one block per CU, as the list kernel's even step (stream_shape)
 	const int vec = c.args.evenVec == 0 ? 1 : c.args.prec == PREC_FP32S ? 2 : c.args.prec == PREC_FP16S ? 4 : 1;
 	const int zq = (g.Z + vec - 1) / vec;
-	const int ebs = zq >= 256 ? 256 : ((zq + 63) / 64) * 64;
+	const int ebs = even_row_threads(zq);
 	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
 	// Defaults measured at 64^3 and 256^3 (duct channel): on lattices larger
 	// than the 256 MiB last-level cache non-temporal accesses (FP64 with the
