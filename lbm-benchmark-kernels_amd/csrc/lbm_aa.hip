@@ -900,6 +900,12 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 	g.X = ld->Dims[0]; g.Y = ld->Dims[1]; g.Z = ld->Dims[2];
 	g.px = ld->PeriodicX; g.py = ld->PeriodicY; g.pz = ld->PeriodicZ;
 	if (g.X <= 0 || g.Y <= 0 || g.Z <= 0) { fail("empty dimensions."); aa_release(A); return; }
+	// the step launches take y and x as grid dimensions
+	if (g.X > 65535 || g.Y > 65535) {
+		fail("aa-soa: x and y extents above 65535 are not supported (launch grid limit), got %d x %d.", g.X, g.Y);
+		aa_release(A);
+		return;
+	}
 	g.gY = g.Y + 2;
 	g.zOff = (int)(128 / elem_bytes(A->c.args.prec));
 	g.zStride = ((g.zOff + g.Z + 1 + g.zOff - 1) / g.zOff) * g.zOff;

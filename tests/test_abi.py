@@ -1,11 +1,13 @@
 """The drop-in boundary without a GPU: libraries load, export every symbol
 the headers declare, struct layouts match the reference's, and the entry
-points fail loudly (no CPU fallback) when no device is present."""
+points fail loudly (no CPU fallback) when no device is present.  One GPU
+test checks a limit Init enforces before it allocates anything."""
 import ctypes as C
 import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import PKG, ROOT
@@ -158,6 +160,16 @@ def test_aa_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match=msg):
         lbm.Kernel("aa-soa", geo, args)
+
+
+@pytest.mark.gpu
+def test_aa_rejects_grid_extents_over_limit(lbm):
+    """aa-soa launches y and x as grid dimensions: a y extent of 65536 is
+    refused at Init, before any allocation or launch."""
+    dims = (1, 65536, 1)
+    geo = lbm.Geometry.from_mask(np.ones(65536, dtype=np.int32), dims)
+    with pytest.raises(RuntimeError, match="65535"):
+        lbm.Kernel("aa-soa", geo)
 
 
 def test_batch_fails_loudly_without_device(lbm):
