@@ -348,8 +348,8 @@ __global__ __launch_bounds__(512) void k_aa_odd(typename Codec::S * pdf, Codec c
 
 // V stored values of consecutive z cells as one access.  The pulls and
 // pushes of the odd step are offset by one cell in z, so the address is
-// aligned to one stored value only; the device's global memory
-// instructions accept that.
+// aligned to one stored value only: 4 bytes for the FP32 pairs, which the
+// device's 8-byte global accesses accept.
 template <class S, int V>
 __device__ __forceinline__ VecS<S, V> ld_run(const S * p)
 {
@@ -365,10 +365,10 @@ __device__ __forceinline__ void st_run(S * p, const VecS<S, V> & t)
 }
 
 // Odd step over groups of V consecutive z cells (FP32 pairs, -vec auto): a
-// group of interior cells (fluid, no solid neighbour, not on the
-// box boundary) moves each direction with one access of V values for the
-// pull and one for the push; any other group goes through aa_odd_cell cell
-// by cell.  Each cell's arithmetic is that of k_aa_odd.
+// group of interior cells (fluid, no solid neighbour, not on the box
+// boundary) moves each direction with one access of V values for the pull
+// and one for the push; any other group goes through aa_odd_cell cell by
+// cell.  Each cell's arithmetic is that of k_aa_odd.
 template <class Codec, int V, bool FORCE_LOAD, bool FORCE_STORE>
 __global__ __launch_bounds__(512) void k_aa_odd_v(typename Codec::S * pdf, Codec cod, const uint32_t * __restrict__ info,
                                                   AaGeom g, TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
