@@ -180,16 +180,43 @@ def batch_mode(args, lbm):
 
 
 def precision_mode(args, lbm):
-    """The 64^3 channel (x periodic, walls on y and z) to steady state in
-    FP64, shifted FP32 and shifted FP16, with aa-soa (the small
-    configuration's kernel) and with list-aa-pv-soa: velocity-field and
-    permeability error against FP64 of the same kernel."""
+    """Steady states in FP64, shifted FP32 and shifted FP16, velocity-field
+    and permeability error against FP64 of the same kernel:
+      * the list-AA workload itself (256^3 porous, list-aa-pv-soa, omega
+        1.2, F 1e-6) until the relative x-momentum change per check is
+        below 1e-7 -- FP16 stops at 1.5x the FP64 step count if it does not
+        get there (its rounding floor);
+      * the 64^3 channel (x periodic, walls on y and z) with aa-soa (the
+        small configuration's kernel) and with list-aa-pv-soa."""
     import numpy as np
     import workloads as W
 
-    dims, omega, F, tol = (64, 64, 64), 1.0, 1e-5, 1e-7
-    geo = lbm.Geometry.from_string("channel", dims, (1, 0, 0))   # -periodic-x, as the reference's driver takes it
+    tol = 1e-7
     cx = np.array([0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0], dtype=float)
+    pgeo = lbm.Geometry.from_string(GEOMETRY, DIMS)
+    porous, pref = {}, None
+    for prec in ("fp64", "fp32s", "fp16s"):
+        cap = args.porous_max_steps if pref is None else int(1.5 * pref[2]) + args.check
+        with lbm.Kernel("list-aa-pv-soa", pgeo, ["-lambda", repr(LAMBDA), "-prec", prec, "-quiet"]) as k:
+            k.set_initial_weights()
+            r = k.run_steady(lbm.case_data(omega=OMEGA, xforce=XFORCE, iterations=0), args.check, tol, cap)
+            u = k.dump() @ cx - XFORCE / 2
+        perm = W.permeability(r.momentumX, pgeo.n_fluid, int(np.prod(DIMS)), OMEGA, XFORCE)
+        rec = {"steps": r.steps, "converged": r.converged, "rel_change": r.relChange,
+               "mflups": round(pgeo.n_fluid * r.steps / r.seconds / 1e6, 1), "permeability": perm,
+               "mass_rel_drift": abs(r.mass - pgeo.n_fluid) / pgeo.n_fluid}
+        if pref is None:
+            pref = (u, perm, r.steps)
+        else:
+            rec["velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u - pref[0]) / np.linalg.norm(pref[0]))
+            rec["permeability_rel_err_vs_fp64"] = abs(perm - pref[1]) / abs(pref[1])
+        del u
+        porous[prec] = rec
+        _log(f"list-aa-pv-soa 256^3 porous {prec}: {rec}")
+    del pref
+
+    dims, omega, F = (64, 64, 64), 1.0, 1e-5
+    geo = lbm.Geometry.from_string("channel", dims, (1, 0, 0))   # -periodic-x, as the reference's driver takes it
     out, fields = {}, {}
     for kernel in ("aa-soa", "list-aa-pv-soa"):
         out[kernel], ref = {}, None
@@ -215,11 +242,12 @@ def precision_mode(args, lbm):
     out["aa_equals_list"] = {p: bool(np.array_equal(fields[("aa-soa", p)], fields[("list-aa-pv-soa", p)]))
                              for p in ("fp64", "fp32s", "fp16s")}
     _log(f"aa-soa == list-aa-pv-soa velocity fields: {out['aa_equals_list']}")
-    aa = out["aa-soa"]
-    return _common("velocity rel. L2 vs FP64 at steady state (64^3 channel, aa-soa, shifted FP16)",
-                   aa["fp16s"]["velocity_rel_l2_vs_fp64"], "rel", aa["fp64"]["steps"],
-                   "aa-soa (and list-aa-pv-soa), 64^3 channel (x periodic, walls y/z), omega 1.0, Lambda 3/16, "
-                   f"F 1e-5, to relative x-momentum change < {tol:g} per {args.check} steps",
+    out["porous256_list"] = porous
+    return _common("velocity rel. L2 vs FP64 at steady state (256^3 porous, list-aa-pv-soa, shifted FP32)",
+                   porous["fp32s"]["velocity_rel_l2_vs_fp64"], "rel", porous["fp64"]["steps"],
+                   f"list-aa-pv-soa, {GEOMETRY} {DIMS[0]}^3, omega {OMEGA}, Lambda 3/16, F {XFORCE:g}, to relative "
+                   f"x-momentum change < {tol:g} per {args.check} steps; also aa-soa and list-aa-pv-soa on the 64^3 "
+                   "channel (x periodic, walls y/z), omega 1.0, F 1e-5",
                    {"higher_is_better": False, "precision": out})
 
 
@@ -347,6 +375,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024, help="batch: sample generator seed")
     ap.add_argument("--max-steps", type=int, default=5000, help="batch: step limit per sample")
     ap.add_argument("--check", type=int, default=100, help="batch/precision: steps between convergence checks")
+    ap.add_argument("--porous-max-steps", type=int, default=60000,
+                    help="precision: step limit of the FP64 256^3 porous steady state")
     ap.add_argument("--tol", type=float, default=1e-6, help="batch: relative change of the x momentum per check")
     ap.add_argument("--baseline-stride", type=int, default=8,
                     help="batch: every n-th sample is also solved alone (the per-request FP64 path)")
