@@ -1,8 +1,10 @@
 """Host side on CPU: geometry generators against the oracle and the live
 reference, the porous generator's contract, and the driver's command line
-and error paths (which exit before touching a device)."""
+and error paths (which exit before touching a device), and bench.py's
+refusal of runs on several GPUs."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -111,3 +113,15 @@ def test_mask_outlives_geometry(lbm):
     want = lbm.Geometry.from_string("channel", (12, 10, 8), (1, 0, 0)).mask
     assert np.array_equal(m, want) and m.shape == (8, 10, 12)
     del junk
+
+
+@pytest.mark.parametrize("argv,env", [(["--gpus", "2"], {}), (["--gpus", "8"], {}), ([], {"WORLD_SIZE": "2"})])
+def test_bench_refuses_several_gpus(argv, env):
+    """bench.py measures one GPU: asked for more it exits with status 2 and a
+    message, before it touches a device, and prints no JSON line."""
+    root = os.path.dirname(PKG)
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py")] + argv, capture_output=True, text=True,
+                       env={**os.environ, **env}, timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
