@@ -11,6 +11,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "lbm-benchmark-kernels_amd")
 ORACLE = os.path.join(ROOT, "oracle")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+# Degenerate all-fluid lattices (dims, periodic): each extent of 1 or 2 once,
+# open faces, and periodic wrap onto the cell itself.
+DEGENERATE_SHAPES = [((1, 4, 5), (1, 0, 0)), ((3, 1, 4), (0, 1, 0)), ((2, 3, 1), (0, 0, 1)), ((1, 1, 6), (1, 1, 0)),
+                     ((2, 2, 2), (0, 0, 0)), ((1, 1, 1), (1, 1, 1)), ((5, 1, 1), (0, 0, 0))]
 for p in (PKG, ORACLE):
     if p not in sys.path:
         sys.path.insert(0, p)

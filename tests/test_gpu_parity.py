@@ -15,7 +15,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import blk_args as _blk_args, blk_tuple as _blk, golden, tiny_cases
+from conftest import DEGENERATE_SHAPES, blk_args as _blk_args, blk_tuple as _blk, golden, tiny_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -300,3 +300,25 @@ def test_index_range_limit(lbm, name):
     geo.ld, geo._keep = ld, None
     with pytest.raises(RuntimeError, match="2\\^32"):
         lbm.Kernel(name, geo)
+
+
+@pytest.mark.parametrize("dims,per", DEGENERATE_SHAPES)
+def test_degenerate_shapes(lbm, oracle, dims, per):
+    """Extents of one and two cells, open and periodic faces, all fluid, 20
+    steps: both kernels bit-exact against the oracle (pinned to the
+    reference on the same shapes) in FP64, and in shifted FP32 / FP16
+    against the oracle's shifted restatements."""
+    mask = np.ones(int(np.prod(dims)), dtype=np.int32)
+    geo = lbm.Geometry.from_mask(mask, dims, per)
+    wants = {("aa-soa", "fp64"): oracle.aa_run(mask, dims, per, 20, 1.3, 1e-5)[0],
+             ("list-aa-pv-soa", "fp64"): oracle.list_run(mask, dims, per, 20, 1.3, 1e-5)[0]}
+    for prec in ("fp32s", "fp16s"):
+        wants[("aa-soa", prec)] = oracle.aa_run_shifted(mask, dims, per, 20, 1.3, 1e-5, prec)
+        wants[("list-aa-pv-soa", prec)] = oracle.list_run_shifted(mask, dims, per, 20, 1.3, 1e-5, prec)[0]
+    for (kernel, prec), want in wants.items():
+        print(f"{kernel} {dims} {per} -prec {prec}", flush=True)
+        with lbm.Kernel(kernel, geo, ["-prec", prec]) as k:
+            k.set_initial_weights()
+            k.run(lbm.case_data(omega=1.3, xforce=1e-5, iterations=20))
+            got = k.dump()
+        assert np.array_equal(got, want), (kernel, prec)
