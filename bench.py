@@ -26,7 +26,9 @@ needs with the run-compressed adjacency (304 B plus 72 B per run and 12 B
 per 64-node wavefront, reported by the library: 317.8 B here) give the
 "moved" figures beside it.  traffic: HBM bytes per odd launch from rocprofv3
 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size
-(profiles/list_odd_porous256_traffic.json), null if absent.  The copy and
+(profiles/list_odd_porous256_traffic.json), null if absent; hbm_pair: the
+measured bytes of an even + odd pair from the same file over this run's
+even + odd launch time (the achieved HBM rate of a step pair).  The copy and
 update-19 bandwidths of the device are measured in the same run as
 yardsticks (the reference rates its kernels against update-19 bandwidth).
 
@@ -436,10 +438,16 @@ def main():
     print(f"bench: yardsticks copy {yard['copy_gbs']:.0f} GB/s update-19 {yard['update19_gbs']:.0f} GB/s",
           file=sys.stderr, flush=True)
 
-    traffic = None
+    traffic, pair = None, None
     if args.prec == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tf = json.load(f)
+        traffic = tf.get("bytes_per_launch")
+        if "even" in tf and "odd" in tf:
+            # measured HBM bytes of an even + odd pair over this run's pair time
+            pb = tf["even"]["bytes_per_launch"] + tf["odd"]["bytes_per_launch"]
+            pair_gbs = pb / ((t.evenMsAvg + t.oddMsAvg) / 1e3) / 1e9
+            pair = {"bytes": pb, "achieved_gbs": round(pair_gbs, 1), "frac": round(pair_gbs / PEAK_GBS, 4)}
 
     base = None
     if not args.no_cpu_baseline:
@@ -466,7 +474,7 @@ def main():
                    "n_fluid": nf, "n_cells_padded": info.nCells, "device": dev.get("name")},
         "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step, run-compressed adjacency)",
                      "achieved": round(odd_gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic, "hbm_pair": pair,
                      "bytes_per_flup": b_odd,
                      "bytes_basis": "list-AA odd balance: 19 loads + 19 stores + 18 int32 adjacency entries per "
                                     "node, adjacency compression not credited",
