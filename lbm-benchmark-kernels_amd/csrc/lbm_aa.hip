@@ -25,7 +25,7 @@
 //
 // Storage: FP64 values, or with -prec fp32s / fp16s the deviations f - w_d
 // through the list kernel's codecs (the narrow even step takes 2 / 4 cells
-// of a z row per thread).  Run-time VTK / statistics output follows the
+// of a z row per thread, the FP32 odd step 2).  Run-time VTK / statistics output follows the
 // reference's call points, including the views after an even step.
 #include "lbm_list.h"      // storage codecs, elem_bytes, kW
 
