@@ -17,16 +17,21 @@
 //
 // Device layout: PDF(d, x, y, z) at
 //   d * plane + ((x + 1) * gY + (y + 1)) * zStride + (z + zOff)
-// gY = Y + 2, plane = (X + 2) * gY * zStride, zOff = 128 B of stored values
-// (16 FP64, 32 FP32 or 64 FP16 values; the ghost z = -1 sits at zOff - 1)
-// and zStride a multiple of zOff, so that interior
-// z rows start on a 128-B line and a wavefront's 64 lanes cover whole
-// lines (four, two or one).
+// gY = Y + 2, plane = (X + 2) * gY * zStride.  Rows with z ghost cells:
+// zOff = 128 B of stored values (16 FP64, 32 FP32 or 64 FP16 values; the
+// ghost z = -1 sits at zOff - 1) and zStride a multiple of zOff, so that
+// interior z rows start on a 128-B line and a wavefront's 64 lanes cover
+// whole lines (four, two or one).  Compact rows (zOff 0, zStride Z), for
+// FP64 and FP32 storage when no fluid cell can reach a z ghost cell (z
+// periodic, or no fluid cell on either z face) and Z is a whole number of
+// 128-B lines: the rows of an x slab are then contiguous, which streams
+// 14 % faster than rows with a 256-B gap at 256^3.
 //
 // Storage: FP64 values, or with -prec fp32s / fp16s the deviations f - w_d
 // through the list kernel's codecs (the narrow even step takes 2 / 4 cells
-// of a z row per thread, the FP32 odd step 2).  Run-time VTK / statistics output follows the
-// reference's call points, including the views after an even step.
+// of a z row per thread, the FP32 odd step 2).  Run-time VTK / statistics
+// output follows the reference's call points, including the views after an
+// even step.
 #include "lbm_list.h"      // storage codecs, elem_bytes, kW
 
 // The program's output functions (the reference's Vtk.o / Kernel.o, or
@@ -196,7 +201,8 @@ __global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec 
 // Even step over V consecutive cells z0 .. z0+V-1 of a z row, for the
 // narrow storage types: one 8-byte access per direction carries V values
 // (V = 2 for FP32, 4 for FP16).  zOff, zStride and plane are multiples of V,
-// so the accesses are aligned.  Cells of the group that are solid or past
+// so the accesses are aligned, and with compact rows Z is too, so no group
+// reaches into the next row.  Cells of the group that are solid or past
 // the row end (ghost z = Z, padding) get their values written back
 // unchanged: the slot that takes direction d of a fluid cell, inv(d), takes
 // the cell's own value inv(d) (a decode/encode round trip is exact).  Each
@@ -906,15 +912,34 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		aa_release(A);
 		return;
 	}
+	const uint64_t nTotal = (uint64_t)g.X * g.Y * g.Z;
+	if (nTotal > 0xffffffffull / 19) { fail("number of PDFs exceed 2^32."); aa_release(A); return; }
+	if (!ld->Lattice) { fail("LatticeDesc without a mask."); aa_release(A); return; }
 	g.gY = g.Y + 2;
-	g.zOff = (int)(128 / elem_bytes(A->c.args.prec));
-	g.zStride = ((g.zOff + g.Z + 1 + g.zOff - 1) / g.zOff) * g.zOff;
+	{
+		const int line = (int)(128 / elem_bytes(A->c.args.prec));   // stored values per 128 B
+		bool zFaceFluid = false;
+		if (!g.pz) {
+			const size_t face = (size_t)g.X * g.Y;
+			for (size_t i = 0; i < face && !zFaceFluid; ++i)
+				zFaceFluid = ld->Lattice[i] != LBM_CELL_OBSTACLE ||
+				             ld->Lattice[(size_t)(g.Z - 1) * face + i] != LBM_CELL_OBSTACLE;
+		}
+		// compact rows: no fluid cell reaches a z ghost cell.  Measured at 256^3
+		// (duct channel): even step FP64 0.94 -> 0.83 ms, FP32 0.47 -> 0.45 ms,
+		// but FP16 0.22 -> 0.23 ms, so FP16 keeps the ghost cells.
+		if (g.Z % line == 0 && !zFaceFluid && elem_bytes(A->c.args.prec) >= 4) {
+			g.zOff = 0;
+			g.zStride = g.Z;
+		} else {
+			g.zOff = line;
+			g.zStride = ((line + g.Z + 1 + line - 1) / line) * line;
+		}
+	}
 	g.zWords = (g.Z + 63) / 64;
 	g.plane = (uint64_t)(g.X + 2) * g.gY * g.zStride;
 	A->nPdf = (size_t)19 * g.plane;
 	A->nFluid = ld->nFluid;
-	const uint64_t nTotal = (uint64_t)g.X * g.Y * g.Z;
-	if (nTotal > 0xffffffffull / 19) { fail("number of PDFs exceed 2^32."); aa_release(A); return; }
 
 	if (!A->c.args.quiet) {
 		printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);

@@ -290,6 +290,32 @@ def test_aa_shifted_rows_over_several_blocks(lbm, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64"] + PRECS)
+def test_aa_row_layouts(lbm, oracle, prec):
+    """aa-soa lays its z rows out without ghost cells (FP64, FP32) when no
+    fluid cell can reach one (z periodic, or no fluid cell on either z face)
+    and Z fills whole 128-byte lines, and with ghost cells otherwise.  64-cell rows with
+    scattered solid cells: open z faces (ghost cells), z periodic and solid
+    z faces (compact rows), each against the oracle, node views bit-exact."""
+    dims = (4, 5, 64)
+    rng = np.random.default_rng(3)
+    base = (rng.random(int(np.prod(dims))) > 0.1).astype(np.int32)
+    walls = base.reshape(64, 5, 4).copy()
+    walls[0] = 0
+    walls[-1] = 0
+    for mask, per in ((base, (0, 0, 0)), (base, (0, 0, 1)), (walls.reshape(-1), (1, 0, 0))):
+        geo = lbm.Geometry.from_mask(mask, dims, per)
+        if prec == "fp64":
+            runs = [30]
+            want = oracle.aa_run(mask, dims, per, 30, 1.3, 2e-5, magic=LAM)[0]
+        else:
+            runs = [20, 10]
+            want = oracle.aa_run_shifted(mask, dims, per, runs, 1.3, 2e-5, prec, magic=LAM)
+        print(f"aa rows {dims} {per} -prec {prec}", flush=True)
+        got = _aa_gpu(lbm, geo, runs, 1.3, 2e-5, ["-prec", prec, "-lambda", repr(LAM)])
+        assert np.array_equal(got, want), per
+
+
+@pytest.mark.parametrize("prec", ["fp64"] + PRECS)
 def test_aa_moments_match_node_sums(lbm, prec):
     """aa-soa's FP64 device sums against sums of its node views, and
     against the list kernel's on the same lattice (no open faces, so the two
