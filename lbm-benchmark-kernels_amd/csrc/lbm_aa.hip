@@ -21,11 +21,11 @@
 // zOff = 128 B of stored values (16 FP64, 32 FP32 or 64 FP16 values; the
 // ghost z = -1 sits at zOff - 1) and zStride a multiple of zOff, so that
 // interior z rows start on a 128-B line and a wavefront's 64 lanes cover
-// whole lines (four, two or one).  Compact rows (zOff 0, zStride Z), for
-// FP64 and FP32 storage when no fluid cell can reach a z ghost cell (z
-// periodic, or no fluid cell on either z face) and Z is a whole number of
-// 128-B lines: the rows of an x slab are then contiguous, which streams
-// 14 % faster than rows with a 256-B gap at 256^3.
+// whole lines (four, two or one).  Compact rows (zOff 0, zStride Z rounded
+// up to whole 128-B lines), for FP64 and FP32 storage when no fluid cell can
+// reach a z ghost cell (z periodic, or no fluid cell on either z face): the
+// rows of an x slab are then contiguous when Z fills whole lines, which
+// streams 14 % faster than rows with a 256-B gap at 256^3.
 //
 // Storage: FP64 values, or with -prec fp32s / fp16s the deviations f - w_d
 // through the list kernel's codecs (the narrow even step takes 2 / 4 cells
@@ -201,8 +201,8 @@ __global__ __launch_bounds__(512) void k_aa_even(typename Codec::S * pdf, Codec 
 // Even step over V consecutive cells z0 .. z0+V-1 of a z row, for the
 // narrow storage types: one 8-byte access per direction carries V values
 // (V = 2 for FP32, 4 for FP16).  zOff, zStride and plane are multiples of V,
-// so the accesses are aligned, and with compact rows Z is too, so no group
-// reaches into the next row.  Cells of the group that are solid or past
+// so the accesses are aligned; a group that reaches past the row end stays
+// in the row's padding (zStride is a multiple of V).  Cells of the group that are solid or past
 // the row end (ghost z = Z, padding) get their values written back
 // unchanged: the slot that takes direction d of a fluid cell, inv(d), takes
 // the cell's own value inv(d) (a decode/encode round trip is exact).  Each
@@ -928,9 +928,9 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		// compact rows: no fluid cell reaches a z ghost cell.  Measured at 256^3
 		// (duct channel): even step FP64 0.94 -> 0.83 ms, FP32 0.47 -> 0.45 ms,
 		// but FP16 0.22 -> 0.23 ms, so FP16 keeps the ghost cells.
-		if (g.Z % line == 0 && !zFaceFluid && elem_bytes(A->c.args.prec) >= 4) {
+		if (!zFaceFluid && elem_bytes(A->c.args.prec) >= 4) {
 			g.zOff = 0;
-			g.zStride = g.Z;
+			g.zStride = (g.Z + line - 1) / line * line;
 		} else {
 			g.zOff = line;
 			g.zStride = ((line + g.Z + 1 + line - 1) / line) * line;

@@ -291,18 +291,22 @@ def test_aa_shifted_rows_over_several_blocks(lbm, oracle, prec):
 
 @pytest.mark.parametrize("prec", ["fp64"] + PRECS)
 def test_aa_row_layouts(lbm, oracle, prec):
-    """aa-soa lays its z rows out without ghost cells (FP64, FP32) when no
-    fluid cell can reach one (z periodic, or no fluid cell on either z face)
-    and Z fills whole 128-byte lines, and with ghost cells otherwise.  64-cell rows with
-    scattered solid cells: open z faces (ghost cells), z periodic and solid
-    z faces (compact rows), each against the oracle, node views bit-exact."""
-    dims = (4, 5, 64)
-    rng = np.random.default_rng(3)
-    base = (rng.random(int(np.prod(dims))) > 0.1).astype(np.int32)
-    walls = base.reshape(64, 5, 4).copy()
-    walls[0] = 0
-    walls[-1] = 0
-    for mask, per in ((base, (0, 0, 0)), (base, (0, 0, 1)), (walls.reshape(-1), (1, 0, 0))):
+    """aa-soa lays its z rows out without ghost cells (FP64, FP32; rows
+    padded to whole 128-byte lines) when no fluid cell can reach one (z
+    periodic, or no fluid cell on either z face), and with ghost cells
+    otherwise.  Rows of 64 and 71 cells with scattered solid cells: open z
+    faces (ghost cells), z periodic and solid z faces (compact rows), each
+    against the oracle, node views bit-exact."""
+    cases = []
+    for nz in (64, 71):
+        dims = (4, 5, nz)
+        rng = np.random.default_rng(3)
+        base = (rng.random(int(np.prod(dims))) > 0.1).astype(np.int32)
+        walls = base.reshape(nz, 5, 4).copy()
+        walls[0] = 0
+        walls[-1] = 0
+        cases += [(dims, base, (0, 0, 0)), (dims, base, (0, 0, 1)), (dims, walls.reshape(-1), (1, 0, 0))]
+    for dims, mask, per in cases:
         geo = lbm.Geometry.from_mask(mask, dims, per)
         if prec == "fp64":
             runs = [30]
