@@ -24,8 +24,8 @@
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
  *   -nt on|off        non-temporal loads/stores in the even step (default on
- *                     for list-aa-pv-soa; for aa-soa off, except shifted
- *                     storage on lattices over 256 MiB: measured faster)
+ *                     for list-aa-pv-soa; for aa-soa on for lattices over
+ *                     256 MiB, off for cache-sized ones: measured faster)
  *   -even-cap on|off|force
  *                     even step in 512-thread blocks, one block per compute
  *                     unit (default on for lattices of at least 16 such
