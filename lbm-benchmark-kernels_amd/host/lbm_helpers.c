@@ -155,10 +155,16 @@ void KernelVerifiy(LatticeDesc * ld, KernelData * kd, CaseData * cd, PdfT * erro
 		slice[y * nZ + z] = u;
 	}
 
+	printf("# Kernel validation: writing profile to flow-profile.dat\n");
 	FILE * fh = fopen("flow-profile.dat", "w");
 	if (!fh) { printf("ERROR: opening file flow-profile.dat failed.\n"); exit(1); }
-	printf("# Kernel validation: writing profile to flow-profile.dat\n");
 	fprintf(fh, "# Flow profile in Z direction. Taken at the middle of the X length (= %d) of total length %d.\n", nZ / 2, nZ);
+	/* the two plotting hints of the reference's file (src/Kernel.c:561-562) */
+	const char * name = "flow-profile.dat";
+	fprintf(fh, "# Plot on terminal: gnuplot -e \"set terminal dumb; plot \\\"%s\\\" u 1:3 t \\\"analytical\\\", "
+	            "\\\"\\\" u 1:4 t \\\"simulation\\\";\"\n", name);
+	fprintf(fh, "# Plot graphically: gnuplot -e \"plot \\\"%s\\\" u 1:3 w linesp t \\\"analytical\\\", "
+	            "\\\"\\\" u 1:4 w linesp t \\\"simulation\\\"; pause -1;\"\n", name);
 	fprintf(fh, "# z coord., radius, analytic, simulation, diff abs, diff rel, undim_analytic, undim_sim\n");
 
 	/* plane Poiseuille between walls half a cell inside the first and last
@@ -182,7 +188,7 @@ void KernelVerifiy(LatticeDesc * ld, KernelData * kd, CaseData * cd, PdfT * erro
 		} else {
 			fprintf(fh, "0.0\t");
 		}
-		fprintf(fh, "%e\t%e\n", ana / uMax, sim / uMax);
+		fprintf(fh, "%e\t%e\t\n", ana / uMax, sim / uMax);
 	}
 	fclose(fh);
 	free(slice);

@@ -131,3 +131,36 @@ def test_vtk_content_decodes_to_node_views(lbm, oracle, tmp_path):
     assert np.array_equal(vel[fl, 0], np.array(ux))
     assert (vel[~fl] == 0).all()
     del keep
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 10), (6, 7, 9)])
+def test_flow_profile_matches_reference(lbm, oracle, tmp_path, dims):
+    """KernelVerifiy (src/Kernel.c:511-625): flow-profile.dat byte for byte
+    and the same deviation, on the -verify geometry (x and y periodic, walls
+    at z) with an even and an odd nY."""
+    ref = _ref_io()
+    fns = (ref.KernelVerifiy, lbm.host_lib().KernelVerifiy)
+    for fn in fns:
+        fn.argtypes = [C.c_void_p] * 4
+        fn.restype = None
+    F = 2e-5
+    geo, nodes = _state(lbm, oracle, "box", dims, (1, 1, 0), 40, F=F)
+    kd, keep = node_view_kd(lbm, geo, nodes)
+    cd = lbm.case_data(omega=1.1, xforce=F, iterations=40)
+    files, norms = [], []
+    old = os.getcwd()
+    for i, fn in enumerate(fns):
+        where = tmp_path / str(i)
+        where.mkdir()
+        os.chdir(where)
+        try:
+            norm = C.c_double(-1.0)
+            fn(C.byref(geo.ld), C.byref(kd), C.byref(cd), C.byref(norm))
+        finally:
+            os.chdir(old)
+        files.append((where / "flow-profile.dat").read_bytes())
+        norms.append(norm.value)
+    assert files[0].count(b"\n") == dims[2] + 4
+    assert files[1] == files[0]
+    assert norms[1] == norms[0] > 0
+    del keep
