@@ -101,10 +101,96 @@ def _common(metric, value, unit, steps, workload, extra):
     return line
 
 
+def _batch_solve(lbm, W, samples, geos, prec, kargs, args):
+    """All samples in one batch of storage type prec, to the stopping rule;
+    setup (list builds, autotuning) and stepping timed separately."""
+    with lbm.Batch(kargs + ["-prec", prec]) as b:
+        t0 = time.perf_counter()
+        ids = [b.add(g, s.omega, s.xforce, args.max_steps) for g, s in zip(geos, samples)]
+        tune = b.autotune(4)
+        setup = time.perf_counter() - t0
+        _log(f"{prec}: {len(ids)} domains set up and tuned in {setup:.1f} s; even {tune.evenBlock}/cap "
+             f"{tune.evenCap}/vec {tune.vec}/nt {tune.nontemporal} odd {tune.oddBlock}/xcd group {tune.oddGroup} "
+             f"groups {tune.groupMB} MiB "
+             f"(pair {tune.pairMs:.3f} ms)")
+        t0 = time.perf_counter()
+        active, rounds = len(ids), 0
+        while active:
+            active = b.step(args.check, args.tol)
+            rounds += 1
+            if rounds % 10 == 0:
+                _log(f"{prec}: {rounds * args.check} steps, {active} active")
+        wall = time.perf_counter() - t0
+        gpu_s, flups = b.stats()
+        st = [b.status(i) for i in ids]
+    perm = [W.permeability(x.momentumX, x.nFluid, s.cells, s.omega, s.xforce) for x, s in zip(st, samples)]
+    rec = {"solves_per_s": round(len(ids) / (setup + wall), 3), "wall_s": round(setup + wall, 3),
+           "run_s": round(wall, 3), "gpu_s": round(gpu_s, 3), "setup_s": round(setup, 2),
+           "mflups": round(flups / wall / 1e6, 1), "converged": sum(x.converged for x in st),
+           "steps_median": W.percentile([x.steps for x in st], 50), "steps_max": max(x.steps for x in st),
+           "tuning": {"even_block": tune.evenBlock, "even_cap": tune.evenCap, "vec": tune.vec,
+                      "odd_block": tune.oddBlock, "even_ms": round(tune.evenMs, 4), "odd_ms": round(tune.oddMs, 4),
+                      "group_mib": tune.groupMB, "pair_ms": round(tune.pairMs, 4), "nontemporal": tune.nontemporal,
+                      "odd_group": tune.oddGroup}}
+    _log(f"{prec}: {rec['solves_per_s']} solves/s ({rec['run_s']} s run + {rec['setup_s']} s setup), "
+         f"{rec['mflups']} MFLUP/s, {rec['converged']} converged")
+    return rec, perm, [x.steps for x in st]
+
+
+def _per_request_fp64(lbm, W, samples, geos, kargs, args, stride=1):
+    """The plain path: each sample alone through Init + LbmHipRunSteady +
+    Deinit, FP64, same stopping rule; Init (list build, allocation) is timed
+    with the run, as the batch's setup is."""
+    sub = list(range(0, len(samples), max(1, stride)))
+    t_all, t_run, perm = 0.0, 0.0, {}
+    for i in sub:
+        s, g = samples[i], geos[i]
+        t0 = time.perf_counter()
+        with lbm.Kernel("list-aa-pv-soa", g, kargs + ["-quiet"]) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            t1 = time.perf_counter()
+            r = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), args.check, args.tol,
+                             args.max_steps)
+            t_run += time.perf_counter() - t1
+        t_all += time.perf_counter() - t0
+        perm[i] = W.permeability(r.momentumX, g.n_fluid, s.cells, s.omega, s.xforce)
+    rate = len(sub) / t_all
+    _log(f"per-request fp64: {len(sub)} samples in {t_all:.1f} s (Init + run + Deinit; run alone {t_run:.1f} s) "
+         f"-> {rate:.2f} solves/s")
+    return rate, t_all, perm, t_run
+
+
+def solves_record(args, lbm):
+    """The batched-solve half of the metric on the 512-sample workload: the
+    shifted-FP32 batch against the plain per-request FP64 path on every
+    sample (setup counted on both sides), and the FP32 permeability error
+    against those FP64 answers."""
+    import workloads as W
+
+    samples = W.batch_samples(args.samples, args.seed)
+    geos = [lbm.Geometry.from_string(s.geometry, s.dims) for s in samples]
+    kargs = ["-lambda", repr(LAMBDA)]
+    b32, perm32, _ = _batch_solve(lbm, W, samples, geos, "fp32s", kargs, args)
+    base, t_base, perm64, t_run = _per_request_fp64(lbm, W, samples, geos, kargs, args)
+    err = [abs(perm32[i] - p) / abs(p) for i, p in perm64.items()]
+    run_only = len(samples) / b32["run_s"] / (len(perm64) / t_run)
+    return {"workload": f"{len(samples)} porous samples (edges log-uniform [32,128], porosity U[0.5,0.9], radius "
+                        f"U[3,8], omega U[0.8,1.6], F 1e-6, Lambda 3/16, seed {args.seed}) to relative x-momentum "
+                        f"change < {args.tol:g} per {args.check} steps or {args.max_steps} steps",
+            "batch_fp32s_solves_per_s": b32["solves_per_s"], "per_request_fp64_solves_per_s": round(base, 3),
+            "ratio": round(b32["solves_per_s"] / base, 3),
+            "ratio_run_only": round(run_only, 3),
+            "ratio_note": "ratio: wall time with setup on both sides (batch: list builds + autotuning; per request: "
+                          "Init + initial state + Deinit); ratio_run_only: stepping alone on both sides",
+            "permeability_rel_err_fp32s_vs_fp64": {"max": max(err), "median": W.percentile(err, 50)},
+            "batch_fp32s": b32, "per_request_fp64_s": round(t_base, 2), "per_request_fp64_run_s": round(t_run, 2)}
+
+
 def batch_mode(args, lbm):
     """Many seeded porous samples of uneven size solved to steady state (or the
-    step limit) in one batch, FP64 and shifted FP32; a stride subset solved
-    one at a time through the single-lattice FP64 path for comparison."""
+    step limit) in one batch, FP64 and shifted FP32, against the plain
+    per-request FP64 path on every sample (setup counted on both sides)."""
     import workloads as W
 
     samples = W.batch_samples(args.samples, args.seed)
@@ -112,59 +198,12 @@ def batch_mode(args, lbm):
     geos = [lbm.Geometry.from_string(s.geometry, s.dims) for s in samples]
     _log(f"{len(samples)} samples generated in {time.perf_counter() - t0:.1f} s, "
          f"{sum(g.n_fluid for g in geos) / 1e6:.1f} M fluid nodes")
-    kargs = ["-lambda", repr(LAMBDA), "-pad", "16384+128"]
-    res = {}
+    kargs = ["-lambda", repr(LAMBDA)]
+    res, perms = {}, {}
     for prec in ("fp64", "fp32s"):
-        with lbm.Batch(kargs + ["-prec", prec]) as b:
-            t0 = time.perf_counter()
-            ids = [b.add(g, s.omega, s.xforce, args.max_steps) for g, s in zip(geos, samples)]
-            setup = time.perf_counter() - t0
-            tune = b.autotune(4)
-            _log(f"{prec}: {len(ids)} domains set up in {setup:.1f} s; tuning even {tune.evenBlock}/cap "
-                 f"{tune.evenCap}/vec {tune.vec}/nt {tune.nontemporal} odd {tune.oddBlock} groups {tune.groupMB} MiB "
-                 f"(pair {tune.pairMs:.3f} ms)")
-            t0 = time.perf_counter()
-            active, rounds = len(ids), 0
-            while active:
-                active = b.step(args.check, args.tol)
-                rounds += 1
-                if rounds % 10 == 0:
-                    _log(f"{prec}: {rounds * args.check} steps, {active} active")
-            wall = time.perf_counter() - t0
-            gpu_s, flups = b.stats()
-            st = [b.status(i) for i in ids]
-            perm = [W.permeability(x.momentumX, x.nFluid, s.cells, s.omega, s.xforce) for x, s in zip(st, samples)]
-            res[prec] = {"solves_per_s": round(len(ids) / wall, 3), "wall_s": round(wall, 3),
-                         "gpu_s": round(gpu_s, 3), "setup_s": round(setup, 2),
-                         "mflups": round(flups / wall / 1e6, 1),
-                         "converged": sum(x.converged for x in st),
-                         "steps_median": W.percentile([x.steps for x in st], 50),
-                         "steps_max": max(x.steps for x in st),
-                         "tuning": {"even_block": tune.evenBlock, "even_cap": tune.evenCap, "vec": tune.vec,
-                                    "odd_block": tune.oddBlock, "even_ms": round(tune.evenMs, 4),
-                                    "odd_ms": round(tune.oddMs, 4), "group_mib": tune.groupMB,
-                                    "pair_ms": round(tune.pairMs, 4), "nontemporal": tune.nontemporal},
-                         "_perm": perm, "_steps": [x.steps for x in st]}
-            _log(f"{prec}: {res[prec]['solves_per_s']} solves/s, {res[prec]['mflups']} MFLUP/s, "
-                 f"{res[prec]['converged']} converged")
-    err = [abs(a - b) / abs(b) for a, b in zip(res["fp32s"]["_perm"], res["fp64"]["_perm"])]
-    # the plain path: one lattice at a time, FP64, same stopping rule
-    sub = list(range(0, len(samples), max(1, args.baseline_stride)))
-    t_run = 0.0
-    for i in sub:
-        s, g = samples[i], geos[i]
-        with lbm.Kernel("list-aa-pv-soa", g, kargs + ["-quiet"]) as k:
-            k.set_initial_weights()
-            k.set_shadow_download(False)
-            cd = lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0)
-            t0 = time.perf_counter()
-            k.run_steady(cd, args.check, args.tol, args.max_steps)
-            t_run += time.perf_counter() - t0
-    base = len(sub) / t_run
-    _log(f"per-request fp64: {len(sub)} samples in {t_run:.1f} s -> {base:.2f} solves/s")
-    for r in res.values():
-        r.pop("_perm")
-        r.pop("_steps")
+        res[prec], perms[prec], _ = _batch_solve(lbm, W, samples, geos, prec, kargs, args)
+    err = [abs(a - b) / abs(b) for a, b in zip(perms["fp32s"], perms["fp64"])]
+    base, t_base, _, t_run = _per_request_fp64(lbm, W, samples, geos, kargs, args, args.baseline_stride)
     v = res["fp32s"]["solves_per_s"]
     return _common(
         f"solves/s ({len(samples)} porous samples, batched list-AA, shifted FP32)", v, "solves/s", args.max_steps,
@@ -172,8 +211,10 @@ def batch_mode(args, lbm):
         f"omega U[0.8,1.6], F 1e-6, Lambda 3/16, seed {args.seed}; to relative x-momentum change < {args.tol:g} "
         f"per {args.check} steps or {args.max_steps} steps",
         {"batch": res,
-         "per_request_fp64": {"solves_per_s": round(base, 3), "samples_timed": len(sub),
-                              "note": "every %d-th sample solved alone through Kernel() (run time only)"
+         "per_request_fp64": {"solves_per_s": round(base, 3), "samples_timed": len(range(0, len(samples),
+                                                                                          args.baseline_stride)),
+                              "seconds": round(t_base, 2), "run_seconds": round(t_run, 2),
+                              "note": "every %d-th sample alone through Init + LbmHipRunSteady + Deinit"
                                       % args.baseline_stride},
          "speedup_fp32s_batch_vs_per_request_fp64": round(v / base, 2),
          "speedup_fp64_batch_vs_per_request_fp64": round(res["fp64"]["solves_per_s"] / base, 2),
@@ -380,8 +421,10 @@ def main():
     ap.add_argument("--porous-max-steps", type=int, default=60000,
                     help="precision: step limit of the FP64 256^3 porous steady state")
     ap.add_argument("--tol", type=float, default=1e-6, help="batch: relative change of the x momentum per check")
-    ap.add_argument("--baseline-stride", type=int, default=8,
+    ap.add_argument("--baseline-stride", type=int, default=1,
                     help="batch: every n-th sample is also solved alone (the per-request FP64 path)")
+    ap.add_argument("--no-solves", action="store_true",
+                    help="list: leave out the batched-solve record (512-sample batch vs per-request FP64)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (its share of the lattice at the real shape); "
@@ -454,6 +497,11 @@ def main():
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
         base = cpu_baseline(mask, DIMS, periodic, nf, args.cpu_steps + (args.cpu_steps & 1))
 
+    solves = None
+    if not args.no_solves and args.prec == "fp64":
+        print("bench: batched solves ...", file=sys.stderr, flush=True)
+        solves = solves_record(args, lbm)
+
     dev = lbm.device_info() or {}
     label = {"fp64": "FP64", "fp32s": "FP32 shifted storage", "fp16s": "FP16 shifted storage, FP32 compute"}[args.prec]
     line = {
@@ -488,6 +536,7 @@ def main():
                      "moved_frac_of_copy": round(moved_gbs / yard["copy_gbs"], 4),
                      "step_average_frac": round((b_even + b_odd) / 2 * nf * steps / loop_s / 1e9 / PEAK_GBS, 4)},
         "cpu_baseline": base,
+        "solves": solves,
     }
     os.dup2(json_fd, 1)
     print(json.dumps(line), flush=True)

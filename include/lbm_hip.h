@@ -224,6 +224,9 @@ typedef struct {
 	                       launch covers all domains (replayed from a graph) */
 	double pairMs;      /* per step pair with the chosen groups (autotune) */
 	int nontemporal;    /* 1: non-temporal loads/stores in the even step (-nt) */
+	int oddGroup;       /* odd step: consecutive chunks each XCD takes per turn
+	                       (a power of two up to 256; 0: 8) -- neighbouring
+	                       chunks of one domain then share an L2 */
 } LbmHipBatchTuning;
 /* params: kernel arguments shared by all domains (-prec, -lambda, -adj, ...). */
 LbmHipBatch * LbmHipBatchCreate(const Parameters * params);
@@ -253,7 +256,7 @@ int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
  * keeps the fastest, then `pairs` step pairs for groups of 0 (none), 64,
  * 128, 192, 256, 384, 512, 768 and 1024 MiB with that shape, then the
  * shapes again under the chosen groups; the lattices are restored
- * afterwards. */
+ * afterwards.  Odd-step shapes include oddGroup 8, 32 and 128. */
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */
 int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);

@@ -945,6 +945,9 @@ void D3Q19AaInit_AaSoAHip(LatticeDesc * ld, KernelData ** kernelData, Parameters
 		printf("# blocking x: %3d y: %3d z: %3d\n", g.X + 2, g.Y + 2, g.Z + 2);
 		printf("# allocating data for %llu LB nodes with padding (%llu bytes = %f MiB for the single lattice)\n",
 		       (unsigned long long)(g.plane), (unsigned long long)aa_bytes(A), aa_bytes(A) / 1024.0 / 1024.0);
+		printf("# z rows: %s, %d values apart, z = 0 at value %d\n", g.zOff ? "with ghost cells" : "compact", g.zStride,
+		       g.zOff);
+		fflush(stdout);
 	}
 
 	int * dMask = nullptr;

@@ -73,13 +73,14 @@ __global__ __launch_bounds__(512) void k_batch_even(const BatchDom<typename Code
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, bool RLE>
 __global__ __launch_bounds__(512) void k_batch_odd(const BatchDom<typename Codec::R> * __restrict__ doms,
                                                    const Chunk * __restrict__ chunks,
-                                                   const uint32_t * __restrict__ count, Codec cod)
+                                                   const uint32_t * __restrict__ count, Codec cod, uint32_t gShift)
 {
 	using R = typename Codec::R;
 	using S = typename Codec::S;
-	// the grid is a multiple of 8 * kXcdGroup blocks: each round of
-	// gridDim.x chunks is handed out in the XCD-grouped order
-	const uint32_t nc = *count, slot = xcd_grouped_block(blockIdx.x);
+	// the grid is a multiple of 8 << gShift blocks: each round of gridDim.x
+	// chunks is handed out in the XCD-grouped order, 2^gShift consecutive
+	// chunks per XCD turn
+	const uint32_t nc = *count, slot = xcd_grouped_block(blockIdx.x, gShift);
 	for (uint32_t base = 0; base < nc; base += gridDim.x) {
 		const uint32_t c = base + slot;
 		if (c >= nc) break;
@@ -236,6 +237,16 @@ struct Batch {
 };
 
 static Batch * as_batch(LbmHipBatch * b) { return reinterpret_cast<Batch *>(b); }
+
+// log2 of the odd step's chunks per XCD turn (LbmHipBatchTuning.oddGroup, a
+// power of two; 0 means kXcdGroup).
+static uint32_t odd_group_shift(const Batch * b)
+{
+	const uint32_t g = b->tune.oddGroup > 0 ? (uint32_t)b->tune.oddGroup : kXcdGroup;
+	uint32_t s = 0;
+	while ((1u << s) < g) ++s;
+	return s;
+}
 
 static float f16_scale(const Batch * b) { return (float)(1 << b->args.f16Shift); }
 
@@ -418,7 +429,7 @@ static bool rebuild_tables(Batch * b)
 	if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&oddPerCU, oddFn, b->tune.oddBlock, 0), "occupancy query")) return false;
 	const uint32_t og = oddPerCU > 0 ? (uint32_t)(oddPerCU * b->nCU) : og0;
 	const uint32_t ne = counts[0] ? counts[0] : 1, no = counts[1] ? counts[1] : 1;
-	const uint32_t groupBlocks = 8 * kXcdGroup;
+	const uint32_t groupBlocks = 8u << odd_group_shift(b);
 	const uint32_t newEven = eg < ne ? eg : ne;
 	const uint32_t newOdd = ((og < no ? og : no) + groupBlocks - 1) / groupBlocks * groupBlocks;
 	for (Batch::Group & g : b->groups) {
@@ -427,7 +438,8 @@ static bool rebuild_tables(Batch * b)
 		g.oGrid = ((og < gno ? og : gno) + groupBlocks - 1) / groupBlocks * groupBlocks;
 	}
 	// the graph keeps its grid: a shrinking chunk list needs no new capture
-	if (newEven > b->evenGrid || newOdd > b->oddGrid) b->graphDirty = true;
+	// (unless the odd grid no longer fits the XCD group size)
+	if (newEven > b->evenGrid || newOdd > b->oddGrid || b->oddGrid % groupBlocks) b->graphDirty = true;
 	if (b->graphDirty) {
 		b->evenGrid = newEven;
 		b->oddGrid = newOdd;
@@ -477,9 +489,10 @@ static void launch_batch_odd(Batch * b, const ChunkRange & c, Codec cod, bool fl
 	using R = typename Codec::R;
 	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
 	const dim3 g(c.oddGrid), t(b->tune.oddBlock);
-	if (fs)      k_batch_odd<Codec, true, true, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod);
-	else if (fl) k_batch_odd<Codec, true, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod);
-	else         k_batch_odd<Codec, false, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod);
+	const uint32_t gs = odd_group_shift(b);
+	if (fs)      k_batch_odd<Codec, true, true, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
+	else if (fl) k_batch_odd<Codec, true, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
+	else         k_batch_odd<Codec, false, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
 }
 
 template <class Codec>
@@ -739,6 +752,7 @@ LbmHipBatch * LbmHipBatchCreate(const Parameters * params)
 	b->tune.vec = b->args.evenVec ? vec_max(b->args.prec) : 1;
 	b->tune.oddBlock = b->args.blockSize;
 	b->tune.groupMB = kDefaultGroupMB;
+	b->tune.oddGroup = (int)kXcdGroup;
 	b->tune.nontemporal = b->args.nontemporal != 0;   // -nt (default on)
 	return reinterpret_cast<LbmHipBatch *>(b);
 }
@@ -906,12 +920,14 @@ int LbmHipBatchSetTuning(LbmHipBatch * handle, const LbmHipBatchTuning * t)
 	const bool okBlock = t->evenBlock >= 64 && t->evenBlock <= 512 && t->evenBlock % 64 == 0 &&
 	                     t->oddBlock >= 64 && t->oddBlock <= 512 && t->oddBlock % 64 == 0;
 	if (!okBlock || (t->vec != 1 && t->vec != 2 && t->vec != 4) || t->vec > vec_max(b->args.prec) ||
-	    t->groupMB < 0 || t->groupMB > (1 << 20)) {
+	    t->groupMB < 0 || t->groupMB > (1 << 20) || t->oddGroup < 0 || t->oddGroup > 256 ||
+	    (t->oddGroup & (t->oddGroup - 1))) {
 		fail("LbmHipBatchSetTuning: blocks must be 64..512 in steps of 64, vec 1, 2 (fp32s) or 4 (fp16s), "
-		     "groupMB 0..2^20.");
+		     "groupMB 0..2^20, oddGroup 0 or a power of two up to 256.");
 		return -1;
 	}
 	b->tune = *t;
+	if (b->tune.oddGroup == 0) b->tune.oddGroup = (int)kXcdGroup;
 	b->tune.evenCap = t->evenCap != 0;
 	b->tune.nontemporal = t->nontemporal != 0;
 	b->tablesDirty = true;
@@ -957,7 +973,10 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 	for (int cap = 1; cap >= 0; --cap)
 		for (int blk : {256, 512})
 			for (int v = vm; v >= 1; v /= 2) evens.push_back({blk, cap, v});
-	const int odds[] = {128, 256, 512};
+	struct OddCand { int block, group; };
+	std::vector<OddCand> odds;
+	for (int o : {128, 256, 512})
+		for (int g : {8, 32, 128}) odds.push_back({o, g});
 	float bestEven = 1e30f, bestOdd = 1e30f;
 	LbmHipBatchTuning best = saved;
 	bool ok = true;
@@ -1005,15 +1024,16 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 		if (!(ok = time_it(true, false, &ms))) break;
 		if (ms < bestEven) { bestEven = ms; best.evenBlock = c.block; best.evenCap = c.cap; best.vec = c.vec; }
 	}
-	for (int o : odds) {
+	for (const OddCand & o : odds) {
 		if (!ok) break;
 		b->tune = base;
-		b->tune.oddBlock = o;
+		b->tune.oddBlock = o.block;
+		b->tune.oddGroup = o.group;
 		b->tablesDirty = true;
 		b->graphDirty = true;
 		float ms = 0.0f;
 		if (!(ok = time_it(false, true, &ms))) break;
-		if (ms < bestOdd) { bestOdd = ms; best.oddBlock = o; }
+		if (ms < bestOdd) { bestOdd = ms; best.oddBlock = o.block; best.oddGroup = o.group; }
 	}
 	// cache-sized groups with the chosen shapes: whole pairs, group by group
 	float bestPair = 1e30f;
@@ -1041,14 +1061,15 @@ int LbmHipBatchAutotune(LbmHipBatch * handle, int pairs, LbmHipBatchTuning * out
 			if (!(ok = time_it(false, false, &ms))) break;
 			if (ms < bestPair) { bestPair = ms; best.evenBlock = c.block; best.evenCap = c.cap; best.vec = c.vec; }
 		}
-		for (int o : odds) {
+		for (const OddCand & o : odds) {
 			if (!ok) break;
 			b->tune = best;
-			b->tune.oddBlock = o;
+			b->tune.oddBlock = o.block;
+			b->tune.oddGroup = o.group;
 			b->tablesDirty = true;
 			float ms = 0.0f;
 			if (!(ok = time_it(false, false, &ms))) break;
-			if (ms < bestPair) { bestPair = ms; best.oddBlock = o; }
+			if (ms < bestPair) { bestPair = ms; best.oddBlock = o.block; best.oddGroup = o.group; }
 		}
 		// non-temporal even-step accesses stream past the cache the groups rely on
 		for (int nt = 0; nt <= 1 && ok; ++nt) {

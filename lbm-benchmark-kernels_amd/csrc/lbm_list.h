@@ -131,6 +131,14 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t b)
 	return ((j / kXcdGroup) * 8 + x) * kXcdGroup + j % kXcdGroup;
 }
 
+// The same order with 2^gShift consecutive blocks per XCD turn (the grid a
+// multiple of 8 << gShift).
+__device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t b, uint32_t gShift)
+{
+	const uint32_t x = b % 8, j = b / 8;
+	return (((j >> gShift) * 8 + x) << gShift) + (j & ((1u << gShift) - 1u));
+}
+
 // ------------------------------------------------------ per-node steps
 // Even step: node n loads its 19 local values, collides and stores
 // direction d into its own slot inv(d) (src/BenchKernelD3Q19ListAaPv.c:380-465).
@@ -213,9 +221,12 @@ __device__ __forceinline__ void even_nodes(typename Codec::S * pdf, Codec cod, u
 struct OddOrder {
 	const uint32_t * table;    // band order, or nullptr
 	uint32_t grouped;          // 1: xcd_grouped_block
+	uint32_t down;             // without a table: launch index b visits grid - 1 - b
 	__device__ __forceinline__ uint32_t block(uint32_t b) const
 	{
-		return table ? table[b] : grouped ? xcd_grouped_block(b) : b;
+		if (table) return table[b];
+		if (down) b = gridDim.x - 1 - b;
+		return grouped ? xcd_grouped_block(b) : b;
 	}
 };
 

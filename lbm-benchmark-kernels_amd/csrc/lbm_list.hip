@@ -602,7 +602,7 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 
 	const int bs = c.args.blockSize;
 	const dim3 grid(L->oddGrid), block((unsigned)bs);
-	const OddOrder order = {L->dOddOrder, c.args.oddOrder == 1 ? 1u : 0u};
+	const OddOrder order = {L->dOddOrder, c.args.oddOrder == 1 ? 1u : 0u, (uint32_t)c.args.oddDown};
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
@@ -817,8 +817,9 @@ static bool list_build_odd_order(ListData * L, hipStream_t s)
 	size_t longest = 0;
 	for (const auto & v : band) longest = v.size() > longest ? v.size() : longest;
 	std::vector<uint32_t> table(8 * longest, 0xffffffffu);
+	const bool down = L->c.args.oddDown != 0;
 	for (uint32_t x = 0; x < 8; ++x)
-		for (size_t j = 0; j < band[x].size(); ++j) table[8 * j + x] = band[x][j];
+		for (size_t j = 0; j < band[x].size(); ++j) table[8 * j + x] = band[x][down ? band[x].size() - 1 - j : j];
 	L->oddGrid = (uint32_t)table.size();
 	if (!dev_alloc((void **)&L->dOddOrder, sizeof(uint32_t) * table.size(), "odd-step block order")) return false;
 	return hip_ok(hipMemcpyAsync(L->dOddOrder, table.data(), sizeof(uint32_t) * table.size(), hipMemcpyHostToDevice, s),
@@ -1113,3 +1114,4 @@ int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, 
 }
 
 }  // extern "C"
+

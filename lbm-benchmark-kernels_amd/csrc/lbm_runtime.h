@@ -51,6 +51,7 @@ struct KernelArgs {
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
 	int oddOrder = 2;      // list: odd-step block order, 0 plain, 1 XCD-grouped, 2 y bands per XCD
+	int oddDown = 0;       // list: odd step visits its node blocks in descending order
 	int launchEvents = 0;  // per-launch timing events (even/odd averages)
 };
 

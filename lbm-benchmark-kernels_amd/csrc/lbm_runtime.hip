@@ -201,6 +201,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			else if (!strcmp(v, "group")) a->oddOrder = 1;
 			else if (!strcmp(v, "band")) a->oddOrder = 2;
 			else return fail("argument -odd-order requires band, group or plain.");
+		} else if (listKernel && (is("-odd-dir") || is("--odd-dir"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -odd-dir requires up or down.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "up")) a->oddDown = 0;
+			else if (!strcmp(v, "down")) a->oddDown = 1;
+			else return fail("argument -odd-dir requires up or down.");
 		} else if (is("-statistics") || is("--statistics")) {
 			a->statistics = 1;
 		} else if (is("-launch-events") || is("--launch-events")) {
@@ -227,7 +233,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-statistics]\n");
-			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy]\n");
+			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain] [-odd-dir up|down]\n");
 			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			printf("  [-launch-events on|off]\n");

@@ -99,7 +99,7 @@ class LbmHipBatchStatus(C.Structure):
 class LbmHipBatchTuning(C.Structure):
     _fields_ = [("evenBlock", C.c_int), ("evenCap", C.c_int), ("vec", C.c_int), ("oddBlock", C.c_int),
                 ("evenMs", C.c_double), ("oddMs", C.c_double), ("groupMB", C.c_int), ("pairMs", C.c_double),
-                ("nontemporal", C.c_int)]
+                ("nontemporal", C.c_int), ("oddGroup", C.c_int)]
 
 
 class LbmHipStreamResult(C.Structure):
@@ -520,8 +520,8 @@ class Batch:
         return t
 
     def set_tuning(self, even_block: int, even_cap: int, vec: int, odd_block: int, group_mb: int = 0,
-                   nontemporal: int = 1):
-        t = LbmHipBatchTuning(even_block, even_cap, vec, odd_block, 0.0, 0.0, group_mb, 0.0, nontemporal)
+                   nontemporal: int = 1, odd_group: int = 8):
+        t = LbmHipBatchTuning(even_block, even_cap, vec, odd_block, 0.0, 0.0, group_mb, 0.0, nontemporal, odd_group)
         self._check(hip_lib().LbmHipBatchSetTuning(self.h, C.byref(t)))
 
     def autotune(self, pairs: int = 4) -> LbmHipBatchTuning:

@@ -15,8 +15,20 @@ and outputs:
   porous256.npz     256^3 porous medium (porous-6-0.8-42): mask SHA-256,
                     adjacency SHA-256 + sampled rows, node views after 20
                     steps at omega 1.2, F 1e-6 (SHA-256 + samples + mass)
+  porous256_500.npz the same lattice after 500 steps (omega 1.2, F 1e-6):
+                    the reference's kernel at its Lambda 1/12 (8 threads)
+                    and the oracle's scalar tree at Lambda 1/12 and 3/16,
+                    each as SHA-256, every 997th node, mass and the sums of
+                    every direction over every x plane; plus the oracle's
+                    largest relative deviation from the reference over the
+                    whole field
+  verify64_12000.npz the reference's analytic check at the size where it
+                    passes (64^3 -verify box, 12000 steps,
+                    omega 1.0, F 1e-5, KernelSetInitialDensity/Velocity):
+                    both kernels' deviation from the Poiseuille profile,
+                    SHA-256 of the node views, every 97th node
 
-Usage: python tests/golden/gen_golden.py [--skip-large]
+Usage: python tests/golden/gen_golden.py [--skip-large] [--only NAME]
 Needs oracle/_ref (built from the reference sources) and liblbmhost.so
 (for the porous generator).  The reference is never imported here as
 Python; it is compiled C called through ctypes.
@@ -36,7 +48,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "lbm-benchmark-kernels_amd"))
-from lbm_checkers import Reference  # noqa: E402
+from lbm_checkers import Oracle, Reference  # noqa: E402
 import lbm_amd  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -63,16 +75,75 @@ def geometry(gstr, dims, periodic):
     return np.array(g.mask.reshape(-1), dtype=np.int32), g.periodic, g.n_fluid
 
 
+def x_plane_sums(dump, mask, nx):
+    """Sum of every direction over the fluid nodes of every x plane
+    (nx x 19), from a node-view dump in mask order."""
+    xs = np.flatnonzero(mask) % nx
+    out = np.zeros((nx, dump.shape[1]))
+    for d in range(dump.shape[1]):
+        out[:, d] = np.bincount(xs, weights=dump[:, d], minlength=nx)
+    return out
+
+
+def porous256_500(ref, manifest):
+    dims, steps, omega, force, every = (256, 256, 256), 500, 1.2, 1e-6, 997
+    mask, ldp, nf = geometry("porous-6-0.8-42", dims, (0, 0, 0))
+    t0 = time.time()
+    out = {"mask_sha": sha(mask), "n_fluid": nf, "steps": steps, "omega": omega, "xforce": force,
+           "sample_every": every}
+    dref, _, rl = ref.list_run(mask, dims, ldp, steps, omega, force, n_threads=8, want_adj=False)
+    orc = Oracle()
+    for tag, lam in (("l12", 1.0 / 12.0), ("l316", 3.0 / 16.0)):
+        d, _ = orc.list_run(mask, dims, ldp, steps, omega, force, magic=lam, n_threads=8, vsize=1)
+        out[f"orc_{tag}_sha"] = sha(d)
+        out[f"orc_{tag}_sample"] = d[::every]
+        out[f"orc_{tag}_mass"] = float(d.sum())
+        out[f"orc_{tag}_xsums"] = x_plane_sums(d, mask, dims[0])
+        if tag == "l12":
+            out["orc_vs_ref_relmax"] = float(np.abs(d - dref).max() / np.abs(dref).max())
+        del d
+    out["ref_sha"] = sha(dref)
+    out["ref_sample"] = dref[::every]
+    out["ref_mass"] = float(dref.sum())
+    out["ref_xsums"] = x_plane_sums(dref, mask, dims[0])
+    out["mass0"] = float(nf)       # f = w: one unit of mass per node
+    np.savez_compressed(os.path.join(OUT, "porous256_500.npz"), **out)
+    manifest["porous256_500"] = {"nFluid": nf, "steps": steps, "orc_vs_ref_relmax": out["orc_vs_ref_relmax"],
+                                 "secs": time.time() - t0}
+    print("porous256_500", nf, out["orc_vs_ref_relmax"], time.time() - t0, flush=True)
+
+
+def verify64(ref, manifest):
+    dims, steps = (64, 64, 64), 12000
+    mask, ldp, nf = geometry("box", dims, (1, 1, 0))
+    t0 = time.time()
+    dl, _, rl = ref.list_run(mask, dims, ldp, steps, 1.0, 1e-5, n_threads=8, init="verify", verify=True,
+                             want_adj=False)
+    da, ra = ref.aa_run(mask, dims, ldp, steps, 1.0, 1e-5, n_threads=8, init="verify", verify=True)
+    np.savez_compressed(os.path.join(OUT, "verify64_12000.npz"), mask_sha=sha(mask), n_fluid=nf, steps=steps,
+                        list_t8_sha=sha(dl), list_t8_sample=dl[::97], list_norm=rl.verifyNorm,
+                        aa_sha=sha(da), aa_sample=da[::97], aa_norm=ra.verifyNorm)
+    manifest["verify64_12000"] = {"nFluid": nf, "list_norm": rl.verifyNorm, "aa_norm": ra.verifyNorm,
+                                  "secs": time.time() - t0}
+    print("verify64_12000", rl.verifyNorm, ra.verifyNorm, time.time() - t0, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--only", choices=["porous256_500", "verify64_12000"])
     args = ap.parse_args()
     ref = Reference("verif")
-    manifest = {}
+    with open(os.path.join(OUT, "manifest.json")) as f:
+        manifest = json.load(f)
     cwd = os.getcwd()
     tmp = tempfile.mkdtemp()
     os.chdir(tmp)          # the reference's verification writes flow-profile.dat
     try:
+        if args.only:
+            {"porous256_500": porous256_500, "verify64_12000": verify64}[args.only](ref, manifest)
+            write_manifest(manifest)
+            return
         for name, gstr, dims, per, kargs in TINY:
             mask, ldp, nf = geometry(gstr, dims, per)
             steps, omega, force = 100, 1.3, 1e-5
@@ -128,8 +199,15 @@ def main():
             manifest["porous256"] = {"nFluid": nf, "nCells": rl.nCells, "runs_t8": rl.nConsecNodes,
                                      "secs": time.time() - t0}
             print("porous256", nf, rl.nCells, time.time() - t0, flush=True)
+            del dl, b
+            porous256_500(ref, manifest)
+            verify64(ref, manifest)
     finally:
         os.chdir(cwd)
+    write_manifest(manifest)
+
+
+def write_manifest(manifest):
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, default=lambda o: list(o) if isinstance(o, tuple) else str(o))
 

@@ -63,8 +63,10 @@ def test_batch_launch_shapes(lbm, prec):
     vmax = {"fp64": 1, "fp32s": 2, "fp16s": 4}[prec]
     want = [_alone(lbm, g, om, F, [40] if prec != "fp16s" else [20, 20], args) for g, om, F in geos]
     # fifth entry: groups of about that many MiB (one or two of these domains each)
+    # seventh entry: odd-step chunks per XCD turn
     shapes = [(512, 1, vmax, 256), (256, 0, 1, 128), (512, 0, vmax, 512), (64, 0, 1, 64), (256, 1, vmax, 256),
-              (512, 1, vmax, 256, 1), (256, 0, 1, 128, 2, 0), (512, 1, vmax, 256, 0, 0)]
+              (512, 1, vmax, 256, 1), (256, 0, 1, 128, 2, 0), (512, 1, vmax, 256, 0, 0),
+              (256, 1, vmax, 256, 0, 1, 128), (512, 0, 1, 128, 1, 1, 32), (256, 1, vmax, 512, 0, 1, 1)]
     for shape in shapes + ["auto"]:
         print(f"batch {prec} shape {shape}", flush=True)
         with lbm.Batch(args + ["-adj", "full"] if shape == (64, 0, 1, 64) else args) as b:
@@ -73,6 +75,7 @@ def test_batch_launch_shapes(lbm, prec):
                 t = b.autotune(2)
                 assert t.evenMs > 0 and t.oddMs > 0 and t.pairMs > 0 and 1 <= t.vec <= vmax
                 assert t.groupMB in (0, 64, 128, 192, 256, 384, 512, 768, 1024)
+                assert t.oddGroup in (8, 32, 128)
             else:
                 b.set_tuning(*shape)
             b.step(20)
@@ -158,3 +161,61 @@ def test_batch_nodes_per_thread_change(lbm, prec):
         runs = [20, 20, 20] if prec == "fp16s" else [60]
         assert np.array_equal(b.dump(c), _alone(lbm, even[0], even[1], even[2], runs, args))
         assert np.array_equal(b.dump(a), _alone(lbm, odd[0], odd[1], odd[2], [20], args))
+
+
+def test_c4_batch_of_512_samples(lbm, oracle):
+    """The batched workload at its configured size: 512 seeded porous
+    samples (edges log-uniform in [32, 128] per axis, porosity U[0.5, 0.9],
+    sphere radius U[3, 8], omega U[0.8, 1.6], F 1e-6), each to a relative
+    x-momentum change <= 1e-6 per 100 steps or 5000 steps, in FP64 and in
+    shifted FP32, autotuned.  Every domain ends converged or at its limit;
+    FP32 permeabilities are within 1e-3 of FP64; every 64th domain equals its
+    lattice run alone through Kernel() for the same steps, bit for bit; the
+    smallest domain equals the oracle in both storage types."""
+    import workloads as W
+
+    samples = W.batch_samples(512, 2024)
+    geos = [lbm.Geometry.from_string(s.geometry, s.dims) for s in samples]
+    limit, check, tol = 5000, 100, 1e-6
+    args = ["-lambda", repr(LAM)]
+    stride = list(range(0, 512, 64))
+    small = min(range(512), key=lambda i: geos[i].n_fluid)
+    res = {}
+    for prec in ("fp64", "fp32s"):
+        print(f"c4 batch {prec}: 512 domains, {sum(g.n_fluid for g in geos)} nodes", flush=True)
+        with lbm.Batch(args + ["-prec", prec]) as b:
+            ids = [b.add(g, s.omega, s.xforce, limit) for g, s in zip(geos, samples)]
+            b.autotune(2)
+            active, rounds = len(ids), 0
+            while active:
+                active = b.step(check, tol)
+                rounds += 1
+                assert rounds <= limit // check + 1
+            st = [b.status(i) for i in ids]
+            dumps = {j: b.dump(ids[j]) for j in stride + [small]}
+        assert all(x.active == 0 for x in st)
+        assert all(x.converged == 1 or x.steps == limit for x in st)
+        assert all(x.converged == 0 or 0 <= x.relChange <= tol for x in st)
+        res[prec] = ([W.permeability(x.momentumX, x.nFluid, s.cells, s.omega, s.xforce)
+                      for x, s in zip(st, samples)], [x.steps for x in st], dumps)
+        print(f"c4 batch {prec}: {sum(x.converged for x in st)} converged, median steps "
+              f"{W.percentile([x.steps for x in st], 50)}", flush=True)
+        for j in stride:
+            g, s = geos[j], samples[j]
+            with lbm.Kernel("list-aa-pv-soa", g, args + ["-prec", prec, "-quiet"]) as k:
+                k.set_initial_weights()
+                k.run(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=st[j].steps))
+                assert np.array_equal(dumps[j], k.dump()), (prec, j)
+        g, s = geos[small], samples[small]
+        m = g.mask.reshape(-1)
+        if prec == "fp64":
+            want, _ = oracle.list_run(m, g.dims, g.periodic, st[small].steps, s.omega, s.xforce, magic=LAM,
+                                      n_threads=8, vsize=1)
+        else:
+            want, _ = oracle.list_run_shifted(m, g.dims, g.periodic, st[small].steps, s.omega, s.xforce, prec,
+                                              magic=LAM)
+        assert np.array_equal(dumps[small], want), prec
+    p64, p32 = np.array(res["fp64"][0]), np.array(res["fp32s"][0])
+    err = np.abs(p32 - p64) / np.abs(p64)
+    print(f"c4 permeability rel. err fp32s vs fp64: max {err.max():.3e} median {np.median(err):.3e}", flush=True)
+    assert (p64 > 0).all() and err.max() <= 1e-3

@@ -13,6 +13,8 @@ Bars:
   * moment sums against sums of the node views, relative 1e-12 (mass) and
     1e-10 (momentum, a difference of nearly equal terms).
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -290,7 +292,7 @@ def test_aa_shifted_rows_over_several_blocks(lbm, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64"] + PRECS)
-def test_aa_row_layouts(lbm, oracle, prec):
+def test_aa_row_layouts(lbm, oracle, prec, capfd):
     """aa-soa lays its z rows out without ghost cells (FP64, FP32; rows
     padded to whole 128-byte lines) when no fluid cell can reach one (z
     periodic, or no fluid cell on either z face), and with ghost cells
@@ -315,8 +317,21 @@ def test_aa_row_layouts(lbm, oracle, prec):
             runs = [20, 10]
             want = oracle.aa_run_shifted(mask, dims, per, runs, 1.3, 2e-5, prec, magic=LAM)
         print(f"aa rows {dims} {per} -prec {prec}", flush=True)
+        capfd.readouterr()
         got = _aa_gpu(lbm, geo, runs, 1.3, 2e-5, ["-prec", prec, "-lambda", repr(LAM)])
+        out = capfd.readouterr().out
         assert np.array_equal(got, want), per
+        # the layout Init chose: compact rows unless a fluid cell reaches a z
+        # ghost cell (open z faces here) or the storage is FP16
+        line = 128 // {"fp64": 8, "fp32s": 4, "fp16s": 2}[prec]
+        nz = dims[2]
+        m = re.search(r"# z rows: (compact|with ghost cells), (\d+) values apart, z = 0 at value (\d+)", out)
+        assert m, out
+        if (per[2] or not mask.reshape(nz, -1)[[0, -1]].any()) and prec != "fp16s":
+            want_layout = ("compact", -(-nz // line) * line, 0)
+        else:
+            want_layout = ("with ghost cells", -(-(line + nz + 1) // line) * line, line)
+        assert (m.group(1), int(m.group(2)), int(m.group(3))) == want_layout, (dims, per, prec)
 
 
 @pytest.mark.parametrize("prec", ["fp64"] + PRECS)

@@ -51,3 +51,50 @@ def test_served_results_equal_alone(lbm, budget_slots):
             cur += b
             peak = max(peak, cur)
         assert peak <= budget
+
+
+def test_c5_poisson_stream_at_500_per_s(lbm, oracle):
+    """Serving at its configured shape: a Poisson stream at 500 requests/s of
+    seeded porous cubes (edge U[32, 192], porosity U[0.5, 0.9], sphere radius
+    U[3, 8], omega U[0.8, 1.6], F 1e-6), each tagged FP64 or shifted FP32, run
+    to a relative x-momentum change < 1e-6 per 100 steps under a 200 GB
+    budget.  Every answer equals the same request solved alone (steps, node
+    views, permeability, bit for bit); the smallest FP64 and the smallest
+    FP32 request equal the oracle at their step counts; p50/p99 latency and
+    solves/s are reported."""
+    n, interval, tol, limit = 12, 100, 1e-6, 20000
+    samples = S.serving_samples(n, 2025)
+    arrivals = S.poisson_arrivals(n, 500.0, 2525)
+    reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arrivals, samples))]
+    assert {r.prec for r in reqs} == {"fp64", "fp32s"}
+    print(f"c5 stream: {n} requests, edges {[r.sample.dims[0] for r in reqs]}", flush=True)
+    be = S.GpuBackend(lbm, KARGS, limit, keep_nodes=True)
+    try:
+        recs = S.serve(reqs, be, int(200e9), interval, tol)
+        summ = S.summarize(recs, be.flups())
+    finally:
+        be.close()
+    print(f"c5 served: {summ}", flush=True)
+    assert summ["rejected"] == 0 and summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
+    smallest = {p: min((r for r in reqs if r.prec == p), key=lambda r: r.sample.cells) for p in ("fp64", "fp32s")}
+    for rec, r in zip(recs, reqs):
+        s = r.sample
+        print(f"c5 request {r.rid} {r.prec} edge {s.dims[0]} steps {rec.steps}", flush=True)
+        assert rec.converged == 1 or rec.steps == limit
+        geo = lbm.Geometry.from_string(s.geometry, s.dims)
+        with lbm.Kernel("list-aa-pv-soa", geo, KARGS + ["-prec", r.prec, "-quiet"]) as k:
+            k.set_initial_weights()
+            a = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), interval, tol, limit)
+            assert (rec.steps, rec.converged) == (a.steps, a.converged)
+            assert np.array_equal(rec.nodes, k.dump())
+            assert rec.permeability == W.permeability(a.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce)
+        if r is smallest[r.prec]:
+            m = geo.mask.reshape(-1)
+            if r.prec == "fp64":
+                want, _ = oracle.list_run(m, geo.dims, geo.periodic, rec.steps, s.omega, s.xforce, magic=0.1875,
+                                          n_threads=8, vsize=1)
+            else:
+                want, _ = oracle.list_run_shifted(m, geo.dims, geo.periodic, rec.steps, s.omega, s.xforce, r.prec,
+                                                  magic=0.1875)
+            assert np.array_equal(rec.nodes, want), r.prec
+        rec.nodes = None

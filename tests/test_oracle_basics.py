@@ -210,3 +210,27 @@ def test_runs_count(oracle):
     assert oracle.list_runs(b.adj, b.n_fluid, 4) == 768
     r1 = oracle.list_runs(b.adj, b.n_fluid, 1)
     assert r1 <= 768
+
+
+def test_duct_flow_converges_to_the_series(lbm, oracle):
+    """The oracle's aa-soa (TRT at Lambda 3/16, half-way bounce-back, Luo
+    force) in square ducts of side W = 16 and 30 (x periodic, walls on y and
+    z) at steady state: the mean velocity (u = sum c_x f - F/2) approaches
+    the textbook square-duct series F W^2 / (12 nu) (1 - 192 / pi^5
+    sum_{n odd} tanh(n pi / 2) / n^5) with a second-order wall error (the
+    relative error falls by (30 / 16)^2 from one size to the next).  A
+    dropped force term, a wrong wall position or a wrong viscosity moves it
+    by far more."""
+    F, om = 1e-5, 1.0
+    cx = np.array([0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0], dtype=float)
+    s = sum(np.tanh(n * np.pi / 2) / n ** 5 for n in range(1, 400, 2))
+    err = {}
+    for W, steps in ((16, 6000), (30, 12000)):
+        dims = (4, W + 2, W + 2)
+        geo = lbm.Geometry.from_string("channel", dims, (1, 0, 0))
+        d, _ = oracle.aa_run(geo.mask.reshape(-1), dims, geo.periodic, steps, om, F, magic=3 / 16)
+        u = float((d @ cx - F / 2).mean())
+        want = F * W * W / (12 * (1 / om - 0.5) / 3) * (1 - 192 / np.pi ** 5 * s)
+        err[W] = u / want - 1
+    assert 0 < err[16] <= 5e-3 and 0 < err[30] <= 1.5e-3
+    assert abs(err[16] / err[30] / (30 / 16) ** 2 - 1) <= 0.1
