@@ -268,6 +268,31 @@ int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);
  *   coords 3*nFluid uint32 */
 int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, uint32_t * coords);
 
+/* The list build of D3Q19ListAaPvInit_PushSoAHip (node numbering, grid,
+ * coords, 18-entry push adjacency; bit-identical to it and to the
+ * reference's, src/BenchKernelD3Q19ListAaPvCommon.c:627-793) into device
+ * buffers the caller owns, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops with preallocated pools, harnesses
+ * that forbid hipMalloc).
+ *   dMask     device, dims[0]*dims[1]*dims[2] int32 cell types in the
+ *             LatticeDesc.Lattice order (x fastest)
+ *   periodic  PeriodicX/Y/Z; blk node-numbering blocks (NULL or 0: whole
+ *             extent, as -blk)
+ *   nCells    PDF node stride the adjacency values encode (d * nCells + m)
+ *   adjStride row stride of dAdj, laid out [18][adjStride]
+ *   capacity  nodes dCoords and dAdj have room for
+ * Outputs (device): dGrid[cells] node id per cell (0xffffffff for
+ * obstacles), dCoords[3 * nFluid], dAdj rows 0..nFluid-1 (entries past
+ * nFluid are not written), *nFluid (host).  Fails, writing nothing, when the
+ * lattice has more than 67,108,608 cells (the scratch limit) or more fluid
+ * cells than capacity, nCells or adjStride.  Runs on the null stream and
+ * returns when done.  Its 1 MiB scratch is a static device array shared by
+ * every call on a GPU: calls that overlap on one GPU race. */
+int LbmHipListBuildNoAlloc(const int * dMask, const int dims[3], const int periodic[3], const int blk[3],
+                           int nCells, int adjStride, int capacity, uint32_t * dGrid, uint32_t * dCoords,
+                           uint32_t * dAdj, int * nFluid);
+
 /* Node views of every non-solid cell of ld, in mask order (x fastest),
  * 19 values each, through kd->GetNode / kd->SetNode. */
 int LbmHipDumpNodes(KernelData * kd, const LatticeDesc * ld, PdfT * out);

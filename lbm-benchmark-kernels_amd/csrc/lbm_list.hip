@@ -37,6 +37,7 @@ void KernelStatistics(KernelData * kd, LatticeDesc * ld, CaseData * cd, int iter
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1115,3 +1116,103 @@ int LbmHipListCopyBuild(const KernelData * kd, uint32_t * adj, uint32_t * grid, 
 
 }  // extern "C"
 
+// ------------------------------------------- list build without allocation
+// LbmHipListBuildNoAlloc runs the list build of D3Q19ListAaPvInit_PushSoAHip
+// (k_num_count, k_scan_once, k_num_assign, k_adjacency) into device buffers
+// the caller owns.  Its only scratch -- the per-block fluid counts of the
+// numbering and their total -- lives in the file-scope array below instead of
+// a hipMalloc'ed buffer.  Each GPU holds its own copy of the array until the
+// process ends; all calls on one GPU share it and leave the last call's
+// contents, so calls that overlap on one GPU race.  Host code reaches the
+// array only through the device address hipGetSymbolAddress returns for the
+// current device (g_scratchDev), never through its name.
+namespace lbm {
+
+constexpr size_t kScratchBytes = size_t(1) << 20;                                      // 1 MiB
+constexpr uint32_t kScratchMaxBlocks = (uint32_t)(kScratchBytes / sizeof(uint32_t)) - 1;   // + the total
+
+static __device__ __align__(256) uint32_t scratch[kScratchBytes / sizeof(uint32_t)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMutex;
+static uint32_t * g_scratchDev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+static uint32_t * scratch_address()
+{
+	int dev = -1;
+	if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+	if (dev < 0 || dev >= kMaxDevices) {
+		fail("LbmHipListBuildNoAlloc: device %d outside the %d devices this library tracks.", dev, kMaxDevices);
+		return nullptr;
+	}
+	std::lock_guard<std::mutex> lock(g_scratchMutex);
+	if (!g_scratchDev[dev]) {
+		void * p = nullptr;
+		if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return nullptr;
+		g_scratchDev[dev] = static_cast<uint32_t *>(p);
+	}
+	return g_scratchDev[dev];
+}
+
+}  // namespace lbm
+
+extern "C" int LbmHipListBuildNoAlloc(const int * dMask, const int dims[3], const int periodic[3], const int blk[3],
+                                      int nCells, int adjStride, int capacity, uint32_t * dGrid, uint32_t * dCoords,
+                                      uint32_t * dAdj, int * nFluid)
+{
+	if (nFluid) *nFluid = 0;
+	if (!dMask || !dims || !periodic || !dGrid || !dCoords || !dAdj || !nFluid) {
+		fail("LbmHipListBuildNoAlloc: null argument.");
+		return -1;
+	}
+	const int X = dims[0], Y = dims[1], Z = dims[2];
+	if (X <= 0 || Y <= 0 || Z <= 0 || 19ull * (uint64_t)X * Y * Z > 0xffffffffull) {
+		fail("number of PDFs exceed 2^32 or empty dimensions.");   // src/Main.c:399-406
+		return -1;
+	}
+	if (nCells < 0 || adjStride < 0 || capacity < 0 || 19ull * (uint64_t)nCells > 0xffffffffull) {
+		fail("LbmHipListBuildNoAlloc: nCells %d, adjStride %d, capacity %d out of range.", nCells, adjStride, capacity);
+		return -1;
+	}
+	int b[3];
+	for (int i = 0; i < 3; ++i) {
+		const int v = blk ? blk[i] : 0;
+		if (v < 0) { fail("LbmHipListBuildNoAlloc: negative block size %d.", v); return -1; }
+		b[i] = v ? v : dims[i];
+	}
+	const uint32_t N = (uint32_t)X * Y * Z;
+	const uint32_t nBlocks = (N + kNumBlock - 1) / kNumBlock;
+	if (nBlocks > kScratchMaxBlocks) {
+		fail("LbmHipListBuildNoAlloc: %u cells exceed the scratch limit of %u cells (%zu bytes).", N,
+		     kScratchMaxBlocks * (uint32_t)kNumBlock, kScratchBytes);
+		return -1;
+	}
+	uint32_t * s = scratch_address();
+	if (!s) return -1;
+
+	hipStream_t st = nullptr;
+	(void)hipGetLastError();
+	const NumGeom g = {X, Y, Z, b[0], b[1], b[2]};
+	k_num_count<<<nBlocks, kNumBlock, 0, st>>>(dMask, g, N, s);
+	k_scan_once<<<1, 1024, 0, st>>>(s, nBlocks, s + nBlocks);
+	if (!hip_ok(hipGetLastError(), "numbering kernels")) return -1;
+	uint32_t total = 0;
+	if (!hip_ok(hipMemcpyAsync(&total, s + nBlocks, sizeof(total), hipMemcpyDeviceToHost, st), "count download")) return -1;
+	if (!hip_ok(hipStreamSynchronize(st), "numbering kernels")) return -1;
+	if (total > (uint32_t)capacity || total > (uint32_t)nCells || total > (uint32_t)adjStride) {
+		fail("LbmHipListBuildNoAlloc: %u fluid cells, room for %d nodes (nCells %d, adjStride %d).", total, capacity,
+		     nCells, adjStride);
+		return -1;
+	}
+	k_num_assign<<<nBlocks, kNumBlock, 0, st>>>(dMask, g, N, s, dGrid, dCoords);
+	if (total > 0) {
+		const AdjGeom ag = {X, Y, Z, periodic[0], periodic[1], periodic[2], total, (uint32_t)nCells,
+		                    (uint32_t)adjStride};
+		k_adjacency<<<(total + 255) / 256, 256, 0, st>>>(dMask, dGrid, dCoords, ag, dAdj);
+	}
+	if (!hip_ok(hipGetLastError(), "adjacency kernels")) return -1;
+	if (!hip_ok(hipStreamSynchronize(st), "adjacency kernels")) return -1;
+	*nFluid = (int)total;
+	return 0;
+}

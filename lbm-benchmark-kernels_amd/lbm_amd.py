@@ -125,7 +125,8 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipListRunSteady", "LbmHipMoments", "LbmHipRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
-               "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait"]
+               "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait",
+               "LbmHipListBuildNoAlloc"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity", "VtkWrite",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -212,6 +213,9 @@ def hip_lib() -> C.CDLL:
                                           C.POINTER(LbmHipSteady)]
         L.LbmHipRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
                                       C.POINTER(LbmHipSteady)]
+        I3 = C.c_int * 3
+        L.LbmHipListBuildNoAlloc.argtypes = [C.c_void_p, I3, I3, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         _libs["hip"] = L
     return _libs["hip"]
 
