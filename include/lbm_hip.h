@@ -60,6 +60,10 @@
  *                     device's 8 XCDs -- band (default): each XCD takes the
  *                     blocks of one eighth of the y range; group: 8
  *                     consecutive blocks per XCD in turn; plain: launch order
+ *   -graph on|off     capture the step launches of a Kernel() call as a HIP
+ *                     graph and replay it while steps, omega and force stay
+ *                     the same (default off; not with -launch-events, VTK
+ *                     output or -statistics, which run the plain loop)
  *   -quiet            no informational lines at Init
  *   -launch-events on|off
  *                     time every even and odd launch with HIP events
@@ -260,6 +264,14 @@ int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */
 int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);
+
+/* Upper bound of the device memory D3Q19ListAaPvInit_PushSoAHip takes for
+ * the lattice ld with these kernel arguments (padding, storage type, ...),
+ * computed on the host without touching the device: every allocation
+ * Init makes, its temporary buffers included, the run-compressed adjacency
+ * at its largest, each rounded up to 2 MiB.  For admission against a
+ * memory budget (the serving loop). */
+int LbmHipListDeviceBytes(const LatticeDesc * ld, const Parameters * params, uint64_t * bytes);
 
 /* Copies the device list build to caller buffers, in the reference's
  * layouts (any pointer may be NULL):

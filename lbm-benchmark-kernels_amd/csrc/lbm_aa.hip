@@ -765,7 +765,16 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
 	bool segmentStart = true;  // the force of the previous step is in the stored values
 	bool synced = true;        // the host copy holds the state of the last launch
-	for (int it = 0; it < steps && A->nFluid > 0; it += 2) {
+	const bool useGraph = c.args.graph && !perLaunch && !vtk && !stats && steps > 0 && A->nFluid > 0;
+	if (useGraph) {
+		// the same launches as the loop below without output, replayed
+		auto record = [&]() {
+			for (int it = 0; it < steps; it += 2) half(force, force && it > 0, force && it + 2 >= steps, it, true, true);
+		};
+		if (!common_graph_run(&c, steps, cd->Omega, cd->XForce, record)) return false;
+		synced = false;
+	}
+	for (int it = 0; it < steps && A->nFluid > 0 && !useGraph; it += 2) {
 		const bool statsEven = stats && due(it), vtkEven = vtk && it % cd->VtkModulus == 0;
 		const bool vtkOdd = vtk && cd->VtkModulus == 1 && it == 0, statsOdd = stats && due(it + 1);
 		const bool mid = statsEven || vtkEven, end = vtkOdd || statsOdd;

@@ -221,12 +221,9 @@ __device__ __forceinline__ void even_nodes(typename Codec::S * pdf, Codec cod, u
 struct OddOrder {
 	const uint32_t * table;    // band order, or nullptr
 	uint32_t grouped;          // 1: xcd_grouped_block
-	uint32_t down;             // without a table: launch index b visits grid - 1 - b
 	__device__ __forceinline__ uint32_t block(uint32_t b) const
 	{
-		if (table) return table[b];
-		if (down) b = gridDim.x - 1 - b;
-		return grouped ? xcd_grouped_block(b) : b;
+		return table ? table[b] : grouped ? xcd_grouped_block(b) : b;
 	}
 };
 

@@ -603,7 +603,7 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 
 	const int bs = c.args.blockSize;
 	const dim3 grid(L->oddGrid), block((unsigned)bs);
-	const OddOrder order = {L->dOddOrder, c.args.oddOrder == 1 ? 1u : 0u, (uint32_t)c.args.oddDown};
+	const OddOrder order = {L->dOddOrder, c.args.oddOrder == 1 ? 1u : 0u};
 	const uint32_t nF = (uint32_t)L->nFluid, nC = (uint32_t)L->nCells;
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
@@ -639,7 +639,19 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	if (!hip_ok(hipEventRecord(c.evLoop[0], c.stream), "hipEventRecord")) return false;
 	bool segmentStart = true;  // the force of the previous step is in the stored values
 	bool synced = true;        // the host copy holds the state of the last launch
-	for (int it = 0; it < steps && L->nFluid > 0; it += 2) {
+	const bool useGraph = c.args.graph && !perLaunch && !out.vtk && !out.stats && steps > 0 && L->nFluid > 0;
+	if (useGraph) {
+		// the same launches as the loop below without output, replayed
+		auto record = [&]() {
+			for (int it = 0; it < steps; it += 2) {
+				even(force && it > 0);
+				odd(force, force && it + 2 >= steps);
+			}
+		};
+		if (!common_graph_run(&c, steps, cd->Omega, cd->XForce, record)) return false;
+		synced = false;
+	}
+	for (int it = 0; it < steps && L->nFluid > 0 && !useGraph; it += 2) {
 		const bool output = out.at(cd, it);
 		synced = false;
 		const bool last = it + 2 >= steps || output;
@@ -818,9 +830,8 @@ static bool list_build_odd_order(ListData * L, hipStream_t s)
 	size_t longest = 0;
 	for (const auto & v : band) longest = v.size() > longest ? v.size() : longest;
 	std::vector<uint32_t> table(8 * longest, 0xffffffffu);
-	const bool down = L->c.args.oddDown != 0;
 	for (uint32_t x = 0; x < 8; ++x)
-		for (size_t j = 0; j < band[x].size(); ++j) table[8 * j + x] = band[x][down ? band[x].size() - 1 - j : j];
+		for (size_t j = 0; j < band[x].size(); ++j) table[8 * j + x] = band[x][j];
 	L->oddGrid = (uint32_t)table.size();
 	if (!dev_alloc((void **)&L->dOddOrder, sizeof(uint32_t) * table.size(), "odd-step block order")) return false;
 	return hip_ok(hipMemcpyAsync(L->dOddOrder, table.data(), sizeof(uint32_t) * table.size(), hipMemcpyHostToDevice, s),
@@ -1214,5 +1225,44 @@ extern "C" int LbmHipListBuildNoAlloc(const int * dMask, const int dims[3], cons
 	if (!hip_ok(hipGetLastError(), "adjacency kernels")) return -1;
 	if (!hip_ok(hipStreamSynchronize(st), "adjacency kernels")) return -1;
 	*nFluid = (int)total;
+	return 0;
+}
+
+// ------------------------------------------------------ device footprint
+// Upper bound of the device memory D3Q19ListAaPvInit_PushSoAHip takes for
+// a lattice, from the same sizes Init derives (padding, strides), without
+// touching the device: every allocation Init makes, the build's temporary
+// buffers included, the run-compressed adjacency at its largest (one run
+// per node) and each allocation rounded up to the allocator's 2 MiB.
+extern "C" int LbmHipListDeviceBytes(const LatticeDesc * ld, const Parameters * params, uint64_t * bytes)
+{
+	if (!ld || !bytes) { fail("LbmHipListDeviceBytes: null argument."); return -1; }
+	*bytes = 0;
+	KernelArgs a;
+	if (!parse_kernel_args(params, &a, true)) return -1;
+	if (ld->Dims[0] <= 0 || ld->Dims[1] <= 0 || ld->Dims[2] <= 0 || ld->nFluid < 0) {
+		fail("LbmHipListDeviceBytes: empty dimensions.");
+		return -1;
+	}
+	const uint64_t N = (uint64_t)ld->Dims[0] * ld->Dims[1] * ld->Dims[2];
+	const uint64_t nF = (uint64_t)ld->nFluid;
+	const long nCells = pad_cells(ld->nFluid, (int)sizeof(PdfT), a.havePad ? &a.pad : nullptr);
+	if (nCells < 0) { fail("LbmHipListDeviceBytes: padding failed."); return -1; }
+	const uint64_t adjStride = (nF + 255) / 256 * 256, nWaves = (nF + 63) / 64;
+	const uint64_t nBlocks = (N + kNumBlock - 1) / kNumBlock, oddBlocks = (nF + a.blockSize - 1) / a.blockSize;
+	const uint64_t sizes[] = {
+		elem_bytes(a.prec) * 19 * (uint64_t)nCells,       // PDFs
+		4 * 18 * adjStride,                                // adjacency
+		4 * N,                                             // grid
+		12 * (nF > 0 ? nF : 1),                            // coords
+		8 * (2 * ((nF + 255) / 256) + 2),                  // moment sums
+		8 * nWaves, 4 * nWaves, 4 * 18 * adjStride,        // run masks, bases, run adjacency (at most one run per node)
+		4 * 8 * (oddBlocks + 8),                           // odd-step block order
+		4 * N, 4 * (nBlocks + 2), 4, 4 * (oddBlocks + 1),  // build temporaries: mask, scan, run count, block rows
+	};
+	const uint64_t page = 2ull << 20;
+	uint64_t total = 0;
+	for (uint64_t s : sizes) total += (s + page - 1) / page * page;
+	*bytes = total;
 	return 0;
 }

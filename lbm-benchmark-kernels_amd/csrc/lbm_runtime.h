@@ -51,8 +51,8 @@ struct KernelArgs {
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
 	int oddOrder = 2;      // list: odd-step block order, 0 plain, 1 XCD-grouped, 2 y bands per XCD
-	int oddDown = 0;       // list: odd step visits its node blocks in descending order
 	int launchEvents = 0;  // per-launch timing events (even/odd averages)
+	int graph = 0;         // replay the step loop of a Kernel() call from a captured HIP graph
 };
 
 // PDF storage of the list kernel.
@@ -106,6 +106,12 @@ struct Common {
 	hipEvent_t evLoop[2] = {nullptr, nullptr};
 	LbmHipTimings t = {};
 	bool (*download)(KernelData * kd) = nullptr;   // device lattice -> host shadow
+	// the step loop of the last Kernel() call, captured (-graph on), and the
+	// values it was captured for
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graphExec = nullptr;
+	int graphSteps = 0;
+	double graphOmega = 0.0, graphForce = 0.0;
 };
 
 // SetNode may change the host shadow only while it holds the device's
@@ -119,6 +125,7 @@ Common * aa_common(KernelData * kd);
 
 bool common_init(Common * c, const Parameters * params, bool listKernel);
 void common_free(Common * c);
+void common_graph_drop(Common * c);
 // Launch shape of a streaming step kernel over `threads` threads.  With the
 // cap, blocks of 512 threads request just over half the CU's LDS, so one
 // block -- 8 wavefronts -- runs per CU: fewer concurrent streams, measured
@@ -134,6 +141,35 @@ StreamShape stream_shape(const Common * c, int defaultBlock, uint64_t threads);
 bool common_events(Common * c, int launches);
 // Fills c->t from the recorded events (stream must be synchronised).
 void common_collect(Common * c, int launches, bool perLaunch);
+
+// Runs the step loop of `steps` steps for (omega, xForce) on c->stream by
+// replaying a HIP graph; `record` issues the loop's kernel launches and is
+// called, under stream capture, only when no graph was captured for these
+// values yet.  The graph holds kernel launches only.
+template <class F>
+bool common_graph_run(Common * c, int steps, double omega, double xForce, F record)
+{
+	if (!c->graphExec || c->graphSteps != steps || c->graphOmega != omega || c->graphForce != xForce) {
+		common_graph_drop(c);
+		(void)hipGetLastError();
+		if (!hip_ok(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
+		record();
+		const hipError_t launchErr = hipGetLastError();
+		const hipError_t endErr = hipStreamEndCapture(c->stream, &c->graph);
+		if (!hip_ok(launchErr, "step kernel launch") || !hip_ok(endErr, "hipStreamEndCapture")) {
+			common_graph_drop(c);
+			return false;
+		}
+		if (!hip_ok(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+			common_graph_drop(c);
+			return false;
+		}
+		c->graphSteps = steps;
+		c->graphOmega = omega;
+		c->graphForce = xForce;
+	}
+	return hip_ok(hipGraphLaunch(c->graphExec, c->stream), "hipGraphLaunch");
+}
 
 // Pinned host buffer (hipHostMalloc) -- used for the host shadow.
 void * host_alloc(size_t bytes);

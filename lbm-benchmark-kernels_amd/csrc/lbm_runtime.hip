@@ -201,12 +201,6 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			else if (!strcmp(v, "group")) a->oddOrder = 1;
 			else if (!strcmp(v, "band")) a->oddOrder = 2;
 			else return fail("argument -odd-order requires band, group or plain.");
-		} else if (listKernel && (is("-odd-dir") || is("--odd-dir"))) {
-			if (i + 1 >= p->nKernelArgs) return fail("argument -odd-dir requires up or down.");
-			const char * v = p->KernelArgs[++i];
-			if (!strcmp(v, "up")) a->oddDown = 0;
-			else if (!strcmp(v, "down")) a->oddDown = 1;
-			else return fail("argument -odd-dir requires up or down.");
 		} else if (is("-statistics") || is("--statistics")) {
 			a->statistics = 1;
 		} else if (is("-launch-events") || is("--launch-events")) {
@@ -215,6 +209,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "on")) a->launchEvents = 1;
 			else if (!strcmp(v, "off")) a->launchEvents = 0;
 			else return fail("argument -launch-events requires on or off.");
+		} else if (is("-graph") || is("--graph")) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -graph requires on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "on")) a->graph = 1;
+			else if (!strcmp(v, "off")) a->graph = 0;
+			else return fail("argument -graph requires on or off.");
 		} else if (is("-quiet") || is("--quiet")) {
 			a->quiet = 1;
 		} else if (is("-even-cap") || is("--even-cap")) {
@@ -232,8 +232,8 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("Kernel parameters:\n");
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
-			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-statistics]\n");
-			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain] [-odd-dir up|down]\n");
+			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-statistics] [-graph on|off]\n");
+			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain]\n");
 			if (listKernel) printf("  [-odd-order band|group|plain]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			printf("  [-launch-events on|off]\n");
@@ -343,8 +343,18 @@ bool common_init(Common * c, const Parameters * params, bool listKernel)
 	return true;
 }
 
+void common_graph_drop(Common * c)
+{
+	if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+	if (c->graph) (void)hipGraphDestroy(c->graph);
+	c->graphExec = nullptr;
+	c->graph = nullptr;
+	c->graphSteps = 0;
+}
+
 void common_free(Common * c)
 {
+	common_graph_drop(c);
 	for (int i = 0; i < c->nEv; ++i) (void)hipEventDestroy(c->ev[i]);
 	free(c->ev);
 	c->ev = nullptr;

@@ -126,7 +126,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
                "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait",
-               "LbmHipListBuildNoAlloc"]
+               "LbmHipListBuildNoAlloc", "LbmHipListDeviceBytes"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity", "VtkWrite",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
                 "KernelSetInitialVelocity", "KernelVerifiy", "KernelAddBodyForce", "GeoCreateByType",
@@ -216,8 +216,24 @@ def hip_lib() -> C.CDLL:
         I3 = C.c_int * 3
         L.LbmHipListBuildNoAlloc.argtypes = [C.c_void_p, I3, I3, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+        L.LbmHipListDeviceBytes.argtypes = [C.POINTER(LatticeDesc), C.POINTER(Parameters), C.POINTER(C.c_uint64)]
         _libs["hip"] = L
     return _libs["hip"]
+
+
+def list_device_bytes(geo: "Geometry", kernel_args: Sequence[str] = ()) -> int:
+    """Upper bound of the device bytes a list-aa-pv-soa Init takes for geo
+    (LbmHipListDeviceBytes; host-only, no device needed)."""
+    L = hip_lib()
+    words = [w.encode() for w in kernel_args]
+    argv = (C.c_char_p * max(1, len(words)))(*words)
+    params = Parameters(0, None, len(words), argv)
+    out = C.c_uint64(0)
+    L.LbmHipSetErrorMode(LBM_HIP_ERRORS_RETURN)
+    L.LbmHipClearError()
+    if L.LbmHipListDeviceBytes(C.byref(geo.ld), C.byref(params), C.byref(out)) != 0:
+        raise RuntimeError(L.LbmHipLastError().decode())
+    return int(out.value)
 
 
 def kernel_table() -> dict:

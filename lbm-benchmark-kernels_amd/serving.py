@@ -2,9 +2,10 @@
 
 Requests (a seeded porous sample and a storage tag, "fp64" or "fp32s")
 arrive on a clock.  Between step intervals the server admits waiting
-requests in arrival order while their estimated device bytes fit the
-budget, builds their lattices (list build on the GPU) and adds them to the
-batch of their storage type; every interval advances all admitted lattices
+requests whose device bytes fit the budget -- first fit in arrival order,
+a request passed over by later ones `max_bypass` times holds back the rest
+until it fits -- builds their lattices (list build on the GPU) and adds
+them to the batch of their storage type; every interval advances all admitted lattices
 of both batches together (the two batches' intervals are launched before
 either is waited for, so their work overlaps on the device), and a lattice
 whose x momentum has settled (relative change per interval <= tol) or that
@@ -77,21 +78,31 @@ def serving_samples(n: int, seed: int, edge=(32, 192), porosity=(0.5, 0.9), radi
 
 
 def device_bytes(sample: W.Sample, prec: str) -> int:
-    """Device memory of one lattice, estimated before it is built: PDFs
-    (19 values per node), full and run-compressed adjacency (72 + ~16 B per
-    node), coordinates (12 B per node) and the cell -> node grid (4 B per
-    cell); fluid nodes estimated from the target porosity."""
+    """Device memory of one lattice, estimated from the sample alone (for
+    backends that cannot size a request exactly): PDFs (19 values per
+    node), full and run-compressed adjacency (72 + ~16 B per node),
+    coordinates (12 B per node) and the cell -> node grid (4 B per cell);
+    fluid nodes estimated from the target porosity.  GpuBackend.size gives
+    the library's upper bound for the built geometry instead."""
     nodes = sample.cells * sample.porosity
     per_node = 19 * (8 if prec == "fp64" else 4) + 72 + 16 + 12
     return int(nodes * per_node + 4 * sample.cells)
 
 
 def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.perf_counter,
-          sleep=time.sleep):
-    """Runs the request stream to completion; returns one Record per request."""
+          sleep=time.sleep, max_bypass: int = 8):
+    """Runs the request stream to completion; returns one Record per request.
+
+    A request's device bytes come from backend.size(request) when the
+    backend has it, else from device_bytes.  Waiting requests are admitted
+    first fit in arrival order; a request that does not fit is passed over
+    by later ones at most max_bypass times (0: strict arrival order), after
+    which nothing behind it is admitted until it fits."""
     pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-    queue = deque()
+    queue = []
     recs = {r.rid: Record(r.rid, r.prec, r.arrival) for r in requests}
+    need_of, passed = {}, {}
+    size = getattr(backend, "size", None)
     in_use = 0
     live = {}                                   # handle -> (request, bytes)
     t0 = clock()
@@ -104,22 +115,34 @@ def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.
         running = bool(live)
         if running:
             backend.launch(interval)
-        while queue:                            # arrival order; the head waits for room
-            r = queue[0]
-            need = device_bytes(r.sample, r.prec)
+        i, blocked = 0, []                      # first fit in arrival order
+        while i < len(queue):
+            r = queue[i]
+            if r.rid not in need_of:
+                need_of[r.rid] = size(r) if size else device_bytes(r.sample, r.prec)
+                passed[r.rid] = 0
+            need = need_of[r.rid]
             if need > budget:
-                queue.popleft()
+                queue.pop(i)
                 rec = recs[r.rid]
                 rec.rejected, rec.bytes, rec.finished = True, need, clock() - t0
                 continue
             if in_use + need > budget:
-                break
-            queue.popleft()
+                if passed[r.rid] >= max_bypass:
+                    break                       # it has waited long enough: reserve room for it
+                blocked.append(r.rid)
+                i += 1
+                continue
+            queue.pop(i)
             h, n_fluid = backend.admit(r)
             in_use += need
             live[h] = (r, need)
             rec = recs[r.rid]
             rec.admitted, rec.bytes, rec.n_fluid = clock() - t0, need, n_fluid
+            for b in blocked:
+                passed[b] += 1
+            if any(passed[b] >= max_bypass for b in blocked):
+                break
         if not running:
             if not live and pending:
                 sleep(max(0.0, pending[0].arrival - (clock() - t0)))
@@ -141,15 +164,29 @@ class GpuBackend:
 
     def __init__(self, lbm, kernel_args=(), max_steps=20000, keep_nodes=False):
         self.lbm = lbm
+        self.kernel_args = list(kernel_args)
         self.max_steps = max_steps
         self.keep_nodes = keep_nodes
-        self.batches = {p: lbm.Batch(list(kernel_args) + ["-prec", p]) for p in PRECS}
+        self.batches = {p: lbm.Batch(self.kernel_args + ["-prec", p]) for p in PRECS}
         self.where = {}                         # handle -> (prec, id)
+        self.geos = {}                          # request id -> geometry built for sizing
         self.next = 0
+
+    def _geometry(self, r: Request):
+        if r.rid not in self.geos:
+            s = r.sample
+            self.geos[r.rid] = self.lbm.Geometry.from_string(s.geometry, s.dims)
+        return self.geos[r.rid]
+
+    def size(self, r: Request) -> int:
+        """Device bytes of the request's lattice: the library's upper bound
+        for its built geometry (exact fluid-node count and padding)."""
+        return self.lbm.list_device_bytes(self._geometry(r), self.kernel_args + ["-prec", r.prec])
 
     def admit(self, r: Request):
         s = r.sample
-        geo = self.lbm.Geometry.from_string(s.geometry, s.dims)
+        geo = self._geometry(r)
+        del self.geos[r.rid]
         i = self.batches[r.prec].add(geo, s.omega, s.xforce, self.max_steps)
         h = self.next
         self.next += 1

@@ -403,3 +403,28 @@ def test_degenerate_shapes(lbm, oracle, dims, per):
             k.run(lbm.case_data(omega=1.3, xforce=1e-5, iterations=20))
             got = k.dump()
         assert np.array_equal(got, want), (kernel, prec)
+
+
+@pytest.mark.parametrize("name", ["list-aa-pv-soa", "aa-soa"])
+def test_graph_replay_equals_plain_loop(lbm, name):
+    """-graph on replays the captured step loop of a Kernel() call; every
+    replay must equal the plain launch loop bit for bit, with host work
+    (GetNode/SetNode through the host copy) between the calls and a new
+    capture when omega or the step count change."""
+    geo = lbm.Geometry.from_string("porous-3-0.7-5", (40, 36, 34))
+    args = ["-lambda", "0.1875", "-quiet"]
+    runs = [(20, 1.2), (20, 1.2), (20, 1.2), (30, 0.9), (30, 0.9), (20, 1.2)]
+    print(f"graph replay {name}", flush=True)
+    with lbm.Kernel(name, geo, args + ["-graph", "on"]) as kg, lbm.Kernel(name, geo, args) as kp:
+        for k in (kg, kp):
+            k.set_initial_weights()
+        x, y, z = [int(v) for v in np.argwhere(geo.mask != 0)[5][::-1]]
+        for i, (steps, omega) in enumerate(runs):
+            for k in (kg, kp):
+                k.run(lbm.case_data(omega=omega, xforce=2e-5, iterations=steps))
+            got, want = kg.dump(), kp.dump()
+            assert np.array_equal(got, want), i
+            # host work between replays: nudge one node the same way in both
+            for k in (kg, kp):
+                v = k.get_node(x, y, z)
+                k.set_node(x, y, z, [a * (1 + 1e-3) for a in v])

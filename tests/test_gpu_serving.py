@@ -29,8 +29,8 @@ def test_served_results_equal_alone(lbm, budget_slots):
     samples = S.serving_samples(6, 11, edge=(20, 36))
     arrivals = [0.0, 0.0, 0.01, 0.02, 0.05, 0.3]
     reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arrivals, samples))]
-    budget = budget_slots * max(S.device_bytes(r.sample, r.prec) for r in reqs)
     be = S.GpuBackend(lbm, KARGS, MAX, keep_nodes=True)
+    budget = budget_slots * max(be.size(r) for r in reqs)
     try:
         recs = S.serve(reqs, be, budget, INTERVAL, TOL)
     finally:
@@ -98,3 +98,30 @@ def test_c5_poisson_stream_at_500_per_s(lbm, oracle):
                                                   magic=0.1875)
             assert np.array_equal(rec.nodes, want), r.prec
         rec.nodes = None
+
+
+def test_device_bytes_bound_holds_on_the_device(lbm):
+    """The admission size (LbmHipListDeviceBytes) against what Init really
+    takes from the device (hipMemGetInfo before and after), for a small and
+    a large lattice in both storage types: never less."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+
+    def free_bytes():
+        f, t = C.c_size_t(), C.c_size_t()
+        assert hip.hipMemGetInfo(C.byref(f), C.byref(t)) == 0
+        return f.value
+
+    warm = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with lbm.Kernel("list-aa-pv-soa", warm, ["-quiet"]):
+        pass
+    for gstr, dims in (("porous-3-0.7-5", (40, 36, 34)), ("porous-6-0.8-42", (160, 150, 140))):
+        geo = lbm.Geometry.from_string(gstr, dims)
+        for prec in ("fp64", "fp32s"):
+            args = KARGS + ["-prec", prec, "-quiet", "-shadow", "lazy"]
+            bound = lbm.list_device_bytes(geo, args)
+            f0 = free_bytes()
+            with lbm.Kernel("list-aa-pv-soa", geo, args):
+                used = f0 - free_bytes()
+            print(f"device bytes {dims} {prec}: used {used / 2**20:.1f} MiB, bound {bound / 2**20:.1f} MiB", flush=True)
+            assert 0 < used <= bound

@@ -74,18 +74,53 @@ def _peak_bytes(recs):
     return peak
 
 
+def _bypassed(recs):
+    """Per request: how many later arrivals were admitted before it."""
+    return [sum(1 for q in recs if q.arrival > r.arrival and q.admitted < r.admitted) for r in recs]
+
+
 @pytest.mark.parametrize("rate", [10, 200])
-def test_budget_order_and_completion(rate):
+@pytest.mark.parametrize("max_bypass", [0, 3, 8])
+def test_budget_order_and_completion(rate, max_bypass):
     reqs = _requests(40, rate)
     budget = 3 * max(S.device_bytes(r.sample, r.prec) for r in reqs)
     clk = Clock()
     be = SimBackend(clk)
-    recs = S.serve(reqs, be, budget, 100, 1e-6, clock=clk, sleep=clk.sleep)
+    recs = S.serve(reqs, be, budget, 100, 1e-6, clock=clk, sleep=clk.sleep, max_bypass=max_bypass)
     assert all(not r.rejected and r.finished >= r.admitted >= r.arrival for r in recs)
     assert _peak_bytes(recs) <= budget
-    # admission in arrival order (the head of the queue waits for room)
-    assert be.admit_order == [r.rid for r in sorted(reqs, key=lambda r: r.arrival)]
+    if max_bypass == 0:
+        # strict arrival order: the head of the queue waits for room
+        assert be.admit_order == [r.rid for r in sorted(reqs, key=lambda r: r.arrival)]
+    # no request is overtaken by more than max_bypass later arrivals
+    assert max(_bypassed(recs)) <= max_bypass
     assert [r.steps for r in recs] == [100 * (1 + r.rid % 5) for r in reqs]
+
+
+def test_small_requests_pass_a_large_head():
+    # a large request at the head of the queue that does not fit yet: the
+    # small ones behind it are admitted first (up to max_bypass of them),
+    # then it gets the room
+    big = W.Sample(0, (120, 120, 120), 6.0, 0.8, 1, 1.0, 1e-6)
+    small = [W.Sample(i, (32, 32, 32), 4.0, 0.8, 1 + i, 1.0, 1e-6) for i in range(1, 9)]
+    running = W.Sample(9, (120, 120, 120), 6.0, 0.8, 99, 1.0, 1e-6)
+    reqs = [S.Request(9, 0.0, running), S.Request(0, 0.001, big)] + \
+           [S.Request(s.index, 0.002 + 0.001 * s.index, s) for s in small]
+    nb = S.device_bytes(big, "fp64")
+    budget = nb + S.device_bytes(running, "fp64") // 2      # the big one waits for the running one
+    out = {}
+    for mb in (0, 4):
+        clk = Clock()
+        be = SimBackend(clk)
+        recs = {r.rid: r for r in S.serve(reqs, be, budget, 100, 1e-6, clock=clk, sleep=clk.sleep, max_bypass=mb)}
+        assert _peak_bytes(list(recs.values())) <= budget
+        out[mb] = (be.admit_order, recs)
+    order0, _ = out[0]
+    assert order0 == [9, 0] + [s.index for s in small]
+    order4, recs4 = out[4]
+    assert order4[:6] == [9, 1, 2, 3, 4, 0]         # four small ones pass, then the head is served
+    assert max(_bypassed(list(recs4.values()))) <= 4
+    assert np.mean([recs4[s.index].latency for s in small[:4]]) < np.mean([out[0][1][s.index].latency for s in small[:4]])
 
 
 def test_continuous_batching_overlaps_requests():
@@ -147,3 +182,22 @@ def test_permeability_of_plane_poiseuille(oracle):
     k_exact = 0.5 * float((zp * (h - zp)).sum()) / nz
     k = W.permeability(jx, d.shape[0], nx * ny * nz, omega, F)
     assert k == pytest.approx(k_exact, rel=1e-6)
+
+
+def test_device_bytes_bound_covers_the_lattice(lbm, oracle):
+    """LbmHipListDeviceBytes (host only): at least the padded PDFs, the
+    adjacency and the grid of the built geometry, more with FP64 than FP32,
+    growing with the padding; the serving loop admits by it."""
+    geo = lbm.Geometry.from_string("porous-3-0.7-5", (40, 36, 34))
+    nf, cells = geo.n_fluid, int(np.prod(geo.dims))
+    got = {}
+    for prec, elem in (("fp64", 8), ("fp32s", 4)):
+        b = lbm.list_device_bytes(geo, ["-prec", prec])
+        nc = oracle.pad_cells(nf)
+        assert b >= elem * 19 * nc + 72 * nf + 4 * cells
+        assert b <= elem * 19 * nc + 3 * 72 * (nf + 255) + 12 * nf + 8 * cells + 13 * (2 << 20)
+        got[prec] = b
+    assert got["fp64"] > got["fp32s"]
+    assert lbm.list_device_bytes(geo, ["-pad", "no"]) < got["fp64"]
+    with pytest.raises(RuntimeError, match="fp64, fp32s or fp16s"):
+        lbm.list_device_bytes(geo, ["-prec", "fp8"])
