@@ -41,6 +41,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -86,7 +87,68 @@ def cpu_baseline(mask, dims, periodic, n_fluid, steps):
         _, _, rv = Reference("verif").list_run(mask, dims, periodic, steps, OMEGA, XFORCE, n_threads=threads,
                                                init="weights", want_adj=False)
         out["verification_build_value"] = round(n_fluid * steps / rv.duration / 1e6, 1)
+    # the same benchmark build with its threads pinned one per core (a child
+    # process: the OpenMP runtime reads the binding when it starts)
+    env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
+    try:
+        child = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(steps)],
+                               env=env, capture_output=True, text=True, timeout=300)
+        out["pinned_value"] = round(json.loads(child.stdout.strip().splitlines()[-1])["value"], 1)
+        out["pinned_env"] = "OMP_PLACES=cores OMP_PROC_BIND=close"
+    except (subprocess.SubprocessError, ValueError, IndexError, KeyError) as e:
+        out["pinned_value"] = None
+        out["pinned_error"] = str(e)[:200]
+    out["host"] = host_topology()
     return out
+
+
+def host_topology():
+    """CPU model, sockets, physical cores and logical CPUs of the host (from
+    /proc/cpuinfo) and the CPUs this process may run on."""
+    info = {"logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        cores, sockets, model = set(), set(), None
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    sockets.add(phys)
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+            sockets.add(phys)
+        info.update({"cpu_model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None})
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline_child(steps):
+    """Child of cpu_baseline: the reference's benchmark build on the workload,
+    its MFLUP/s as one JSON line (threads and binding from the environment)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import lbm_amd as lbm
+    from lbm_checkers import Reference
+
+    geo = lbm.Geometry.from_string(GEOMETRY, DIMS)
+    mask = np.ascontiguousarray(geo.mask.reshape(-1), dtype=np.int32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    sys.stdout.flush()
+    keep = os.dup(1)
+    os.dup2(2, 1)                      # the C libraries' report lines go to stderr
+    _, _, res = Reference("bench").list_run(mask, DIMS, geo.periodic, steps, OMEGA, XFORCE, n_threads=threads,
+                                            init="weights", want_adj=False)
+    os.write(keep, (json.dumps({"value": geo.n_fluid * steps / res.duration / 1e6}) + "\n").encode())
 
 
 def _log(msg):
@@ -423,9 +485,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="batch: relative change of the x momentum per check")
     ap.add_argument("--baseline-stride", type=int, default=1,
                     help="batch: every n-th sample is also solved alone (the per-request FP64 path)")
+    ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-solves", action="store_true",
                     help="list: leave out the batched-solve record (512-sample batch vs per-request FP64)")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_baseline_child)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (its share of the lattice at the real shape); "
               "runs on several GPUs are not part of this build.", file=sys.stderr)
@@ -521,6 +587,7 @@ def main():
                                f"x periodic, omega 1.2, Lambda 3/16, Luo force 1e-6, {label}",
                    "n_fluid": nf, "n_cells_padded": info.nCells, "device": dev.get("name")},
         "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step, run-compressed adjacency)",
+                     "scope": "frac is the dominant kernel's (the odd step); step_average_frac is the whole step's",
                      "achieved": round(odd_gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic, "hbm_pair": pair,
                      "bytes_per_flup": b_odd,

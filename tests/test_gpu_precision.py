@@ -198,6 +198,51 @@ def test_porous256_steady_state_fp32(lbm):
     assert abs(kp - k_ref) / k_ref < 1e-4
 
 
+def _channel64_steady(lbm, prec):
+    """The small configuration's channel (64^3, x periodic, walls on y and
+    z, aa-soa, omega 1.0, F 1e-5, Lambda 3/16) to a relative x-momentum
+    change < 1e-7 per 100 steps; (converged, steps, velocity field,
+    permeability)."""
+    geo = lbm.Geometry.from_string("channel", (64, 64, 64), (1, 0, 0))
+    omega, F = 1.0, 1e-5
+    with lbm.Kernel("aa-soa", geo, ["-prec", prec, "-lambda", repr(LAM), "-quiet"]) as k:
+        k.set_initial_weights()
+        r = k.run_steady(lbm.case_data(omega=omega, xforce=F, iterations=0), 100, 1e-7, 80000)
+        u, kp = _ux_perm(k.dump(), 64 ** 3, omega, F)
+    print(f"channel64 {prec}: {r.steps} steps, converged {r.converged}, permeability {kp:.6e}", flush=True)
+    return r.converged, r.steps, u, kp
+
+
+def test_channel64_steady_state_precisions(lbm):
+    """The precision configuration (shifted storage on the small
+    configuration's channel, run to steady state): FP32 converges and stays
+    within 1e-4 of FP64 in velocity and permeability (measured 1.2e-5).
+    FP16 converges too but misses the 1e-3 permeability bar: binary16 keeps
+    11 significant bits of each stored deviation (relative 4.9e-4, whatever
+    the power-of-two scale), while the per-step force increment and the
+    viscous part of the deviations are of the same order, so the rounding
+    of every store biases the steady state -- by 4.5e-2 in permeability
+    here (pinned between 1e-2 and 1e-1; the strict bar is the xfail below)."""
+    c64, _, u64, k64 = _channel64_steady(lbm, "fp64")
+    assert c64
+    c32, _, u32, k32 = _channel64_steady(lbm, "fp32s")
+    assert c32
+    assert np.linalg.norm(u32 - u64) / np.linalg.norm(u64) < 1e-4 and abs(k32 - k64) / k64 < 1e-4
+    c16, _, u16, k16 = _channel64_steady(lbm, "fp16s")
+    assert c16
+    err = abs(k16 - k64) / k64
+    assert 1e-2 < err < 1e-1
+    assert 1e-2 < np.linalg.norm(u16 - u64) / np.linalg.norm(u64) < 1e-1
+
+
+@pytest.mark.xfail(strict=True, reason="binary16 storage of the deviations biases the channel's steady state "
+                                       "by 4.5e-2 in permeability (11 significant bits per stored value)")
+def test_channel64_fp16_meets_the_permeability_bar(lbm):
+    _, _, _, k64 = _channel64_steady(lbm, "fp64")
+    _, _, _, k16 = _channel64_steady(lbm, "fp16s")
+    assert abs(k16 - k64) / k64 <= 1e-3
+
+
 # ---- aa-soa with shifted storage ---------------------------------------
 AA_CASES = ["fluid6", "blocks2_12", "box8", "channel10x9x7", "porous16"]
 
