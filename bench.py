@@ -34,7 +34,17 @@ yardsticks (the reference rates its kernels against update-19 bandwidth).
 
 cpu_baseline: the reference's own list-aa-pv-soa (benchmark build, compiled
 from its sources into oracle/_ref) on the same geometry, OpenMP on the host
-cores, a bounded sample of steps; falls back to the C restatement.
+cores, a bounded sample of steps, unpinned and pinned one thread per core,
+with the host topology; falls back to the C restatement.
+
+solves / serving (FP64 runs, unless --no-solves): the other half of the
+metric, timed in the same run.  solves: 512 seeded porous samples (edges
+log-uniform in [32,128]) to steady state in one shifted-FP32 batch against
+the plain per-request FP64 path on every sample, setup counted on both sides,
+and the FP32 permeability error against those FP64 answers.  serving: a
+Poisson stream (500 req/s, edges U[32,192], mixed and all-FP32 tags) through
+the continuous-batching server, p50/p99 latency and solves/s against the
+plain path queued in arrival order.
 """
 from __future__ import annotations
 
@@ -356,6 +366,73 @@ def precision_mode(args, lbm):
                    {"higher_is_better": False, "precision": out})
 
 
+def _plain_service(lbm, W, samples, kargs, interval, tol, max_steps):
+    """The plain path: each request alone through geometry, Init, initial
+    state, LbmHipRunSteady and Deinit in FP64; service times, FP64
+    permeabilities and node updates."""
+    service, perm64, flups = [], [], 0.0
+    for s, _ in samples:
+        t0 = time.perf_counter()
+        geo = lbm.Geometry.from_string(s.geometry, s.dims)
+        with lbm.Kernel("list-aa-pv-soa", geo, kargs + ["-quiet"]) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            r = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), interval, tol, max_steps)
+        service.append(time.perf_counter() - t0)
+        perm64.append(W.permeability(r.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce))
+        flups += geo.n_fluid * r.steps
+    _log(f"plain fp64 path: {len(samples)} requests, {sum(service):.1f} s of service")
+    return service, perm64, flups
+
+
+def serving_record(lbm, n=16, rate=500.0, seed=2025, tol=1e-6, interval=100, max_steps=20000, budget=200e9):
+    """The serving half of the metric on a short stream of the configured
+    shape: n Poisson arrivals at `rate`, porous cubes of edge U[32,192],
+    FP64 or shifted-FP32 tag, through the continuous-batching server under
+    `budget` bytes; against the plain path (each request alone in FP64,
+    queued in arrival order: measured service times, Lindley's recursion)."""
+    import serving as S
+    import workloads as W
+
+    samples = S.serving_samples(n, seed)
+    kargs = ["-lambda", repr(LAMBDA)]
+    service, perm64, flups_plain = _plain_service(lbm, W, samples, kargs, interval, tol, max_steps)
+    arr = S.poisson_arrivals(n, rate, seed + int(rate))
+    reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arr, samples))]
+    be = S.GpuBackend(lbm, kargs, max_steps)
+    try:
+        recs = S.serve(reqs, be, int(budget), interval, tol)
+        served = S.summarize(recs, be.flups())
+    finally:
+        be.close()
+    fin = S.fifo_latencies(arr, service)
+    lat = [f - a for f, a in zip(fin, arr)]
+    span = fin[-1] - arr[0]
+    plain = {"p50_latency_s": round(W.percentile(lat, 50), 4), "p99_latency_s": round(W.percentile(lat, 99), 4),
+             "solves_per_s": round(n / span, 3), "mflups": round(flups_plain / span / 1e6, 1)}
+    err = [abs(r.permeability - p) / abs(p) for r, p in zip(recs, perm64) if r.prec == "fp32s" and not r.rejected]
+    served = {k: (float(v) if hasattr(v, "item") else v) for k, v in served.items()}
+    served["permeability_rel_err_fp32s_vs_fp64_max"] = max(err) if err else None
+    # the same stream with every request tagged shifted FP32
+    reqs32 = [S.Request(i, a, s, "fp32s") for i, (a, (s, _)) in enumerate(zip(arr, samples))]
+    be = S.GpuBackend(lbm, kargs, max_steps)
+    try:
+        recs32 = S.serve(reqs32, be, int(budget), interval, tol)
+        served32 = S.summarize(recs32, be.flups())
+    finally:
+        be.close()
+    served32 = {k: (float(v) if hasattr(v, "item") else v) for k, v in served32.items()}
+    served32["permeability_rel_err_vs_fp64_max"] = max(abs(r.permeability - p) / abs(p) for r, p in zip(recs32, perm64))
+    plain = {k: float(v) for k, v in plain.items()}
+    _log(f"serving: mixed {served} | all fp32s {served32} | plain {plain}")
+    return {"workload": f"{n} requests, Poisson {rate:g}/s (seed {seed}), porous cubes of edge U[32,192], "
+                        f"porosity U[0.5,0.9], radius U[3,8], omega U[0.8,1.6], F 1e-6, FP64 or shifted-FP32 tag, "
+                        f"to relative x-momentum change < {tol:g} per {interval} steps, {budget / 1e9:g} GB budget",
+            "served": served, "served_all_fp32s": served32, "plain_fp64": plain,
+            "solves_speedup": round(served["solves_per_s"] / plain["solves_per_s"], 3),
+            "solves_speedup_all_fp32s": round(served32["solves_per_s"] / plain["solves_per_s"], 3)}
+
+
 def serve_mode(args, lbm):
     """Poisson streams of permeability requests (porous cubes, edge U[32,192],
     FP64 or shifted-FP32 tag) through the continuous-batching server, one
@@ -368,18 +445,7 @@ def serve_mode(args, lbm):
     samples = S.serving_samples(args.requests, args.seed)
     kargs = ["-lambda", repr(LAMBDA), "-pad", "16384+128"]
     max_steps, tol, interval = 20000, args.tol, args.check
-    service, perm64, flups_plain = [], [], 0.0
-    for s, _ in samples:
-        t0 = time.perf_counter()
-        geo = lbm.Geometry.from_string(s.geometry, s.dims)
-        with lbm.Kernel("list-aa-pv-soa", geo, kargs + ["-quiet"]) as k:
-            k.set_initial_weights()
-            k.set_shadow_download(False)
-            r = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), interval, tol, max_steps)
-        service.append(time.perf_counter() - t0)
-        perm64.append(W.permeability(r.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce))
-        flups_plain += geo.n_fluid * r.steps
-    _log(f"plain fp64 path: {len(samples)} requests, {sum(service):.1f} s of service")
+    service, perm64, flups_plain = _plain_service(lbm, W, samples, kargs, interval, tol, max_steps)
     out = {}
     for rate in rates:
         arr = S.poisson_arrivals(len(samples), rate, args.seed + int(rate))
@@ -487,7 +553,8 @@ def main():
                     help="batch: every n-th sample is also solved alone (the per-request FP64 path)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-solves", action="store_true",
-                    help="list: leave out the batched-solve record (512-sample batch vs per-request FP64)")
+                    help="list: leave out the batched-solve and serving records (512-sample batch and a Poisson "
+                         "request stream, each against the per-request FP64 path)")
     args = ap.parse_args()
     if args.cpu_baseline_child:
         cpu_baseline_child(args.cpu_baseline_child)
@@ -563,10 +630,12 @@ def main():
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
         base = cpu_baseline(mask, DIMS, periodic, nf, args.cpu_steps + (args.cpu_steps & 1))
 
-    solves = None
+    solves = serving = None
     if not args.no_solves and args.prec == "fp64":
         print("bench: batched solves ...", file=sys.stderr, flush=True)
         solves = solves_record(args, lbm)
+        print("bench: served stream ...", file=sys.stderr, flush=True)
+        serving = serving_record(lbm)
 
     dev = lbm.device_info() or {}
     label = {"fp64": "FP64", "fp32s": "FP32 shifted storage", "fp16s": "FP16 shifted storage, FP32 compute"}[args.prec]
@@ -604,6 +673,7 @@ def main():
                      "step_average_frac": round((b_even + b_odd) / 2 * nf * steps / loop_s / 1e9 / PEAK_GBS, 4)},
         "cpu_baseline": base,
         "solves": solves,
+        "serving": serving,
     }
     os.dup2(json_fd, 1)
     print(json.dumps(line), flush=True)
