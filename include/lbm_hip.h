@@ -225,7 +225,9 @@ typedef struct {
 	                       each group through a whole interval before the next
 	                       (its lattices stay in the last-level cache; default
 	                       256); 0, or all domains within one group: every
-	                       launch covers all domains (replayed from a graph) */
+	                       launch covers all domains.  Either way an
+	                       interval is replayed from captured graphs, one
+	                       per group */
 	double pairMs;      /* per step pair with the chosen groups (autotune) */
 	int nontemporal;    /* 1: non-temporal loads/stores in the even step (-nt) */
 	int oddGroup;       /* odd step: consecutive chunks each XCD takes per turn
@@ -264,6 +266,11 @@ int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */
 int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);
+/* Graph bookkeeping of the intervals: the launch ranges of the last interval
+ * (1, or the number of groups), graphs captured and graph replays so far.
+ * A range keeps its graph while the step count, the launch shapes and its
+ * device tables stay; domains leaving or joining only rewrite the tables. */
+int LbmHipBatchGraphStats(LbmHipBatch * b, int * slots, long long * captures, long long * replays);
 
 /* Upper bound of the device memory D3Q19ListAaPvInit_PushSoAHip takes for
  * the lattice ld with these kernel arguments (padding, storage type, ...),

@@ -10,9 +10,11 @@
 //     chunk one block's worth of consecutive nodes of one domain, so small
 //     domains fill the GPU together and the per-domain constants (omega,
 //     force, pointers, sizes) come from a device table;
-//   * the step loop of one check interval captured once as a HIP graph and
-//     replayed; the chunk lists live in device memory, so domains that finish
-//     leave the graph valid (the host rewrites the lists between replays);
+//   * the step loop of one check interval captured as HIP graphs -- one for
+//     the whole chunk list, or one per cache-sized group of domains -- and
+//     replayed; the chunk lists and each launch's chunk range live in device
+//     memory, so domains that finish or join leave the graphs valid (the host
+//     rewrites the lists and ranges between replays);
 //   * FP64 per-domain moments after every interval (segmented reduction in a
 //     fixed order) and a per-domain convergence test on the x momentum;
 //   * an autotuner over block sizes, even-step occupancy and nodes per lane.
@@ -51,17 +53,19 @@ struct Chunk {
 };
 
 // ------------------------------------------------------------- kernels
-// Persistent grids: block b takes chunks b, b + gridDim.x, ... below *count.
+// Persistent grids over the chunk range {first, count} read from `range`:
+// block b takes chunks first + b, first + b + gridDim.x, ... of the range.
 template <class Codec, int V, bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_batch_even(const BatchDom<typename Codec::R> * __restrict__ doms,
                                                     const Chunk * __restrict__ chunks,
-                                                    const uint32_t * __restrict__ count, Codec cod)
+                                                    const uint32_t * __restrict__ range, Codec cod)
 {
 	using R = typename Codec::R;
 	using S = typename Codec::S;
-	const uint32_t nc = *count;
+	const Chunk * __restrict__ list = chunks + range[0];
+	const uint32_t nc = range[1];
 	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
-		const Chunk ch = chunks[c];
+		const Chunk ch = list[c];
 		const BatchDom<R> & D = doms[ch.dom];
 		const uint32_t n0 = ch.node0 + threadIdx.x * V;
 		if (n0 >= D.nFluid) continue;
@@ -73,18 +77,19 @@ __global__ __launch_bounds__(512) void k_batch_even(const BatchDom<typename Code
 template <class Codec, bool FORCE_LOAD, bool FORCE_STORE, bool RLE>
 __global__ __launch_bounds__(512) void k_batch_odd(const BatchDom<typename Codec::R> * __restrict__ doms,
                                                    const Chunk * __restrict__ chunks,
-                                                   const uint32_t * __restrict__ count, Codec cod, uint32_t gShift)
+                                                   const uint32_t * __restrict__ range, Codec cod, uint32_t gShift)
 {
 	using R = typename Codec::R;
 	using S = typename Codec::S;
 	// the grid is a multiple of 8 << gShift blocks: each round of gridDim.x
 	// chunks is handed out in the XCD-grouped order, 2^gShift consecutive
 	// chunks per XCD turn
-	const uint32_t nc = *count, slot = xcd_grouped_block(blockIdx.x, gShift);
+	const Chunk * __restrict__ list = chunks + range[0];
+	const uint32_t nc = range[1], slot = xcd_grouped_block(blockIdx.x, gShift);
 	for (uint32_t base = 0; base < nc; base += gridDim.x) {
 		const uint32_t c = base + slot;
 		if (c >= nc) break;
-		const Chunk ch = chunks[c];
+		const Chunk ch = list[c];
 		const BatchDom<R> & D = doms[ch.dom];
 		const uint32_t n = ch.node0 + threadIdx.x;
 		if (n >= D.nFluid) continue;
@@ -198,12 +203,12 @@ struct Batch {
 	std::vector<BatchEntry> e;
 	LbmHipBatchTuning tune = {};
 
-	// device tables (sized on rebuild; reallocation invalidates the graph)
+	// device tables (sized on rebuild; reallocation invalidates the graphs)
 	void * dDoms = nullptr;
 	size_t domsBytes = 0;
 	Chunk * dEven = nullptr, * dOdd = nullptr, * dMom = nullptr;
 	size_t evenCapN = 0, oddCapN = 0, momCapN = 0;
-	uint32_t * dCount = nullptr;  // [0] even chunks, [1] odd chunks
+	uint32_t * dCount = nullptr;  // {0, even chunks, 0, odd chunks}: the ranges of all chunks
 	uint32_t * dSeg = nullptr;
 	size_t segCapN = 0;
 	double * dPartial = nullptr, * dSums = nullptr;
@@ -213,7 +218,7 @@ struct Batch {
 	std::vector<int> activeIds;    // domain id of moment segment i
 	std::vector<double> hSums;
 	bool tablesDirty = true;
-	uint32_t evenGrid = 1, oddGrid = 1;
+	uint32_t evenGrid = 1, oddGrid = 1;   // persistent grids over all chunks
 	size_t evenLds = 0;
 
 	// tune.groupMB > 0: the active domains in groups of about that many bytes
@@ -221,13 +226,24 @@ struct Batch {
 	// before the next, so that a group's lattices stay in the last-level cache
 	struct Group { uint32_t e0, e1, o0, o1, eGrid, oGrid; };
 	std::vector<Group> groups;
-	uint32_t * dGroupCount = nullptr;  // [2g] even, [2g+1] odd chunks of group g
+	uint32_t * dGroupRange = nullptr;  // [4g .. 4g+3]: first even chunk, even chunks, first odd chunk, odd chunks
 	size_t groupCapN = 0;
 
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t exec = nullptr;
-	int graphSteps = 0;
-	bool graphDirty = true;
+	// One captured graph per launch range (slot 0 the whole chunk list, or
+	// slot g group g): `steps` steps over that range.  A slot is replayed
+	// while its range pointer, step count and grids still hold; the range
+	// contents change freely between replays.
+	struct Slot {
+		hipGraph_t graph = nullptr;
+		hipGraphExec_t exec = nullptr;
+		const uint32_t * range = nullptr;
+		uint32_t eGrid = 0, oGrid = 0;
+		int steps = 0;
+	};
+	std::vector<Slot> slots;
+	bool graphDirty = true;        // drop every slot's graph before the next interval
+	size_t lastSlots = 0;
+	long long captures = 0, replays = 0;
 	int tableVec = 1;              // nodes per thread the even chunk list was cut for
 
 	double gpuMs = 0.0, flups = 0.0;
@@ -362,7 +378,7 @@ static bool rebuild_tables(Batch * b)
 			bytes += db;
 		}
 		if (cur.e1 > cur.e0) b->groups.push_back(cur);
-		// one group is the whole chunk list: the captured graph serves it
+		// one group is the whole chunk list
 		if (b->groups.size() == 1) b->groups.clear();
 	}
 	bool moved = false;
@@ -382,13 +398,13 @@ static bool rebuild_tables(Batch * b)
 	cap = b->sumCapN;
 	if (!grow((void **)&b->dSums, &cap, sumsN, sizeof(double), "moment sums", &moved)) return false;
 	b->sumCapN = cap;
-	if (!b->dCount && !hip_ok(hipMalloc((void **)&b->dCount, 2 * sizeof(uint32_t)), "allocating chunk counts")) return false;
+	if (!b->dCount && !hip_ok(hipMalloc((void **)&b->dCount, 4 * sizeof(uint32_t)), "allocating chunk counts")) return false;
 	if (!b->groups.empty() &&
-	    !grow((void **)&b->dGroupCount, &b->groupCapN, 2 * b->groups.size(), sizeof(uint32_t), "group chunk counts", &moved))
+	    !grow((void **)&b->dGroupRange, &b->groupCapN, 4 * b->groups.size(), sizeof(uint32_t), "group chunk ranges", &moved))
 		return false;
 	if (moved) b->graphDirty = true;
 
-	const uint32_t counts[2] = {(uint32_t)b->hEven.size(), (uint32_t)b->hOdd.size()};
+	const uint32_t counts[4] = {0, (uint32_t)b->hEven.size(), 0, (uint32_t)b->hOdd.size()};
 	hipStream_t s = b->stream;
 	if (!doms.empty() && !hip_ok(hipMemcpyAsync(b->dDoms, doms.data(), doms.size(), hipMemcpyHostToDevice, s), "table upload")) return false;
 	if (!b->hEven.empty() && !hip_ok(hipMemcpyAsync(b->dEven, b->hEven.data(), sizeof(Chunk) * b->hEven.size(), hipMemcpyHostToDevice, s), "chunk upload")) return false;
@@ -396,14 +412,16 @@ static bool rebuild_tables(Batch * b)
 	if (!b->hMom.empty() && !hip_ok(hipMemcpyAsync(b->dMom, b->hMom.data(), sizeof(Chunk) * b->hMom.size(), hipMemcpyHostToDevice, s), "chunk upload")) return false;
 	if (!hip_ok(hipMemcpyAsync(b->dSeg, b->hSeg.data(), sizeof(uint32_t) * b->hSeg.size(), hipMemcpyHostToDevice, s), "segment upload")) return false;
 	if (!hip_ok(hipMemcpyAsync(b->dCount, counts, sizeof(counts), hipMemcpyHostToDevice, s), "count upload")) return false;
-	std::vector<uint32_t> gCounts;
+	std::vector<uint32_t> gRanges;
 	for (const Batch::Group & g : b->groups) {
-		gCounts.push_back(g.e1 - g.e0);
-		gCounts.push_back(g.o1 - g.o0);
+		gRanges.push_back(g.e0);
+		gRanges.push_back(g.e1 - g.e0);
+		gRanges.push_back(g.o0);
+		gRanges.push_back(g.o1 - g.o0);
 	}
-	if (!gCounts.empty() &&
-	    !hip_ok(hipMemcpyAsync(b->dGroupCount, gCounts.data(), sizeof(uint32_t) * gCounts.size(), hipMemcpyHostToDevice, s),
-	            "count upload"))
+	if (!gRanges.empty() &&
+	    !hip_ok(hipMemcpyAsync(b->dGroupRange, gRanges.data(), sizeof(uint32_t) * gRanges.size(), hipMemcpyHostToDevice, s),
+	            "range upload"))
 		return false;
 	if (!hip_ok(hipStreamSynchronize(s), "table upload")) return false;
 
@@ -428,46 +446,44 @@ static bool rebuild_tables(Batch * b)
 	                   : (const void *)k_batch_odd<CodecF16, true, true, true>;
 	if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&oddPerCU, oddFn, b->tune.oddBlock, 0), "occupancy query")) return false;
 	const uint32_t og = oddPerCU > 0 ? (uint32_t)(oddPerCU * b->nCU) : og0;
-	const uint32_t ne = counts[0] ? counts[0] : 1, no = counts[1] ? counts[1] : 1;
+	const uint32_t ne = counts[1] ? counts[1] : 1, no = counts[3] ? counts[3] : 1;
 	const uint32_t groupBlocks = 8u << odd_group_shift(b);
-	const uint32_t newEven = eg < ne ? eg : ne;
-	const uint32_t newOdd = ((og < no ? og : no) + groupBlocks - 1) / groupBlocks * groupBlocks;
+	b->evenGrid = eg < ne ? eg : ne;
+	b->oddGrid = ((og < no ? og : no) + groupBlocks - 1) / groupBlocks * groupBlocks;
 	for (Batch::Group & g : b->groups) {
 		const uint32_t gne = g.e1 - g.e0, gno = g.o1 - g.o0;
 		g.eGrid = eg < gne ? eg : gne;
 		g.oGrid = ((og < gno ? og : gno) + groupBlocks - 1) / groupBlocks * groupBlocks;
 	}
-	// the graph keeps its grid: a shrinking chunk list needs no new capture
-	// (unless the odd grid no longer fits the XCD group size)
-	if (newEven > b->evenGrid || newOdd > b->oddGrid || b->oddGrid % groupBlocks) b->graphDirty = true;
-	if (b->graphDirty) {
-		b->evenGrid = newEven;
-		b->oddGrid = newOdd;
-	}
 	b->tablesDirty = false;
 	return true;
 }
 
-// The chunks one launch walks: a range of the even and odd chunk lists, the
-// device counts of the range and the persistent grid sizes.
+// The chunks one launch walks: the even and odd chunk lists, the device
+// words {first, count} of the range taken from each, and the persistent grid
+// sizes the range needs now.
 struct ChunkRange {
 	const Chunk * even;
 	const Chunk * odd;
-	const uint32_t * evenCount;
-	const uint32_t * oddCount;
+	const uint32_t * evenRange;
+	const uint32_t * oddRange;
 	uint32_t evenGrid, oddGrid;
 };
 
 static ChunkRange all_chunks(const Batch * b)
 {
-	return {b->dEven, b->dOdd, b->dCount, b->dCount + 1, b->evenGrid, b->oddGrid};
+	return {b->dEven, b->dOdd, b->dCount, b->dCount + 2, b->evenGrid, b->oddGrid};
 }
 
 static ChunkRange group_chunks(const Batch * b, size_t g)
 {
 	const Batch::Group & G = b->groups[g];
-	return {b->dEven + G.e0, b->dOdd + G.o0, b->dGroupCount + 2 * g, b->dGroupCount + 2 * g + 1, G.eGrid, G.oGrid};
+	return {b->dEven, b->dOdd, b->dGroupRange + 4 * g, b->dGroupRange + 4 * g + 2, G.eGrid, G.oGrid};
 }
+
+// Launch range of slot g: the whole chunk list, or group g.
+static size_t slot_count(const Batch * b) { return b->groups.empty() ? 1 : b->groups.size(); }
+static ChunkRange slot_chunks(const Batch * b, size_t g) { return b->groups.empty() ? all_chunks(b) : group_chunks(b, g); }
 
 template <class Codec, int V>
 static void launch_batch_even(Batch * b, const ChunkRange & c, Codec cod, bool fl)
@@ -477,10 +493,10 @@ static void launch_batch_even(Batch * b, const ChunkRange & c, Codec cod, bool f
 	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
 	const dim3 g(c.evenGrid), t(b->tune.evenBlock);
 	const bool nt = b->tune.nontemporal != 0;
-	if (fl && nt)  k_batch_even<Codec, V, true, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
-	else if (fl)   k_batch_even<Codec, V, true, false><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
-	else if (nt)   k_batch_even<Codec, V, false, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
-	else           k_batch_even<Codec, V, false, false><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenCount, cod);
+	if (fl && nt)  k_batch_even<Codec, V, true, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenRange, cod);
+	else if (fl)   k_batch_even<Codec, V, true, false><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenRange, cod);
+	else if (nt)   k_batch_even<Codec, V, false, true><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenRange, cod);
+	else           k_batch_even<Codec, V, false, false><<<g, t, b->evenLds, b->stream>>>(doms, c.even, c.evenRange, cod);
 }
 
 template <class Codec, bool RLE>
@@ -490,9 +506,9 @@ static void launch_batch_odd(Batch * b, const ChunkRange & c, Codec cod, bool fl
 	const BatchDom<R> * doms = (const BatchDom<R> *)b->dDoms;
 	const dim3 g(c.oddGrid), t(b->tune.oddBlock);
 	const uint32_t gs = odd_group_shift(b);
-	if (fs)      k_batch_odd<Codec, true, true, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
-	else if (fl) k_batch_odd<Codec, true, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
-	else         k_batch_odd<Codec, false, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddCount, cod, gs);
+	if (fs)      k_batch_odd<Codec, true, true, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddRange, cod, gs);
+	else if (fl) k_batch_odd<Codec, true, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddRange, cod, gs);
+	else         k_batch_odd<Codec, false, false, RLE><<<g, t, 0, b->stream>>>(doms, c.odd, c.oddRange, cod, gs);
 }
 
 template <class Codec>
@@ -534,39 +550,67 @@ static void launch_range_steps(Batch * b, const ChunkRange & c, int steps)
 	}
 }
 
-// `steps` steps of every active domain: all chunks per launch, or group by
-// group with every group taking all the steps before the next one.
+// `steps` steps of every active domain, launched directly: all chunks per
+// launch, or group by group with every group taking all the steps before the
+// next one.
 static void launch_steps(Batch * b, int steps)
 {
-	if (b->groups.empty()) {
-		launch_range_steps(b, all_chunks(b), steps);
-		return;
-	}
-	for (size_t g = 0; g < b->groups.size(); ++g) launch_range_steps(b, group_chunks(b, g), steps);
+	for (size_t g = 0; g < slot_count(b); ++g) launch_range_steps(b, slot_chunks(b, g), steps);
 }
 
-static void drop_graph(Batch * b)
+static void drop_slot(Batch::Slot & s)
 {
-	if (b->exec) (void)hipGraphExecDestroy(b->exec);
-	if (b->graph) (void)hipGraphDestroy(b->graph);
-	b->exec = nullptr;
-	b->graph = nullptr;
-	b->graphSteps = 0;
+	if (s.exec) (void)hipGraphExecDestroy(s.exec);
+	if (s.graph) (void)hipGraphDestroy(s.graph);
+	s = Batch::Slot();
 }
 
-// The launches of `steps` steps recorded as one graph (kernel nodes only).
-static bool capture(Batch * b, int steps)
+static void drop_graphs(Batch * b)
 {
-	drop_graph(b);
+	for (Batch::Slot & s : b->slots) drop_slot(s);
+	b->slots.clear();
+}
+
+// The launches of `steps` steps over slot g's range recorded as one graph
+// (kernel nodes only).
+static bool capture_slot(Batch * b, size_t g, int steps)
+{
+	Batch::Slot & s = b->slots[g];
+	drop_slot(s);
+	const ChunkRange c = slot_chunks(b, g);
 	(void)hipGetLastError();
 	if (!hip_ok(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return false;
-	launch_steps(b, steps);
+	launch_range_steps(b, c, steps);
 	const hipError_t launchErr = hipGetLastError();
-	const hipError_t endErr = hipStreamEndCapture(b->stream, &b->graph);
-	if (!hip_ok(launchErr, "batch step launch") || !hip_ok(endErr, "hipStreamEndCapture")) { drop_graph(b); return false; }
-	if (!hip_ok(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate")) { drop_graph(b); return false; }
-	b->graphSteps = steps;
-	b->graphDirty = false;
+	const hipError_t endErr = hipStreamEndCapture(b->stream, &s.graph);
+	if (!hip_ok(launchErr, "batch step launch") || !hip_ok(endErr, "hipStreamEndCapture")) { drop_slot(s); return false; }
+	if (!hip_ok(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0), "hipGraphInstantiate")) { drop_slot(s); return false; }
+	s.range = c.evenRange;
+	s.eGrid = c.evenGrid;
+	s.oGrid = c.oddGrid;
+	s.steps = steps;
+	++b->captures;
+	return true;
+}
+
+// Every slot's graph ready for `steps` steps over its current range: kept
+// when it was captured for the same range words and step count with grids at
+// least as large as the range needs now (a persistent grid larger than its
+// chunk range only leaves blocks idle), captured again otherwise.
+static bool ready_graphs(Batch * b, int steps)
+{
+	if (b->graphDirty) {
+		drop_graphs(b);
+		b->graphDirty = false;
+	}
+	const size_t n = slot_count(b);
+	if (b->slots.size() < n) b->slots.resize(n);
+	for (size_t g = 0; g < n; ++g) {
+		const Batch::Slot & s = b->slots[g];
+		const ChunkRange c = slot_chunks(b, g);
+		if (s.exec && s.range == c.evenRange && s.steps == steps && s.eGrid >= c.evenGrid && s.oGrid >= c.oddGrid) continue;
+		if (!capture_slot(b, g, steps)) return false;
+	}
 	return true;
 }
 
@@ -603,8 +647,8 @@ static bool batch_moments_launch(Batch * b)
 	              "moment download");
 }
 
-// Starts one interval of `steps` for all active domains (graph replay, then
-// the moment kernels); returns without waiting.
+// Starts one interval of `steps` for all active domains (graph replays, one
+// per slot, then the moment kernels); returns without waiting.
 static bool batch_launch(Batch * b, int steps)
 {
 	if (b->pendingSteps) return fail("LbmHipBatchLaunch: the previous interval has not been waited for.");
@@ -617,18 +661,13 @@ static bool batch_launch(Batch * b, int steps)
 	}
 	if (!any || steps <= 0) return true;
 	if (b->tablesDirty && !rebuild_tables(b)) return false;
-	const bool grouped = !b->groups.empty();
-	if (!grouped && (b->graphDirty || b->graphSteps != steps)) {
-		if (!capture(b, steps)) return false;
-	}
+	if (!ready_graphs(b, steps)) return false;
 	(void)hipGetLastError();
 	if (!hip_ok(hipEventRecord(b->ev[0], b->stream), "hipEventRecord")) return false;
-	if (grouped) {
-		// groups change with the active set; launched directly (2 x steps per group)
-		launch_steps(b, steps);
-		if (!hip_ok(hipGetLastError(), "batch step launch")) return false;
-	} else if (!hip_ok(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch")) {
-		return false;
+	b->lastSlots = slot_count(b);
+	for (size_t g = 0; g < b->lastSlots; ++g) {
+		if (!hip_ok(hipGraphLaunch(b->slots[g].exec, b->stream), "hipGraphLaunch")) return false;
+		++b->replays;
 	}
 	if (!hip_ok(hipEventRecord(b->ev[1], b->stream), "hipEventRecord")) return false;
 	if (!batch_moments_launch(b)) return false;
@@ -680,7 +719,7 @@ static bool batch_step(Batch * b, int steps, double tol)
 
 static void batch_free_tables(Batch * b)
 {
-	drop_graph(b);
+	drop_graphs(b);
 	(void)hipFree(b->dDoms);
 	(void)hipFree(b->dEven);
 	(void)hipFree(b->dOdd);
@@ -689,7 +728,7 @@ static void batch_free_tables(Batch * b)
 	(void)hipFree(b->dSeg);
 	(void)hipFree(b->dPartial);
 	(void)hipFree(b->dSums);
-	(void)hipFree(b->dGroupCount);
+	(void)hipFree(b->dGroupRange);
 }
 
 static bool idle(const Batch * b, const char * who)
@@ -941,6 +980,16 @@ int LbmHipBatchStats(LbmHipBatch * handle, double * gpuSeconds, double * flups)
 	if (!b) { fail("LbmHipBatchStats: invalid batch."); return -1; }
 	if (gpuSeconds) *gpuSeconds = b->gpuMs / 1e3;
 	if (flups) *flups = b->flups;
+	return 0;
+}
+
+int LbmHipBatchGraphStats(LbmHipBatch * handle, int * slots, long long * captures, long long * replays)
+{
+	Batch * b = as_batch(handle);
+	if (!b) { fail("LbmHipBatchGraphStats: invalid batch."); return -1; }
+	if (slots) *slots = (int)b->lastSlots;
+	if (captures) *captures = b->captures;
+	if (replays) *replays = b->replays;
 	return 0;
 }
 

@@ -125,7 +125,7 @@ HIP_SYMBOLS = ["D3Q19ListAaPvInit_PushSoAHip", "D3Q19ListAaPvDeinit_PushSoAHip",
                "LbmHipListRunSteady", "LbmHipMoments", "LbmHipRunSteady", "LbmHipBatchCreate", "LbmHipBatchDestroy", "LbmHipBatchAdd",
                "LbmHipBatchRemove", "LbmHipBatchKernelData", "LbmHipBatchStatusOf", "LbmHipBatchStep",
                "LbmHipBatchRun", "LbmHipBatchSync", "LbmHipBatchGetTuning", "LbmHipBatchSetTuning",
-               "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchLaunch", "LbmHipBatchWait",
+               "LbmHipBatchAutotune", "LbmHipBatchStats", "LbmHipBatchGraphStats", "LbmHipBatchLaunch", "LbmHipBatchWait",
                "LbmHipListBuildNoAlloc", "LbmHipListDeviceBytes"]
 HOST_SYMBOLS = ["D3Q19_X", "D3Q19_Y", "D3Q19_Z", "D3Q19_INV", "D3Q19_NAMES", "KernelDensity", "VtkWrite",
                 "KernelStatistics", "KernelStatisticsAdv", "KernelSetInitialDensity",
@@ -209,6 +209,8 @@ def hip_lib() -> C.CDLL:
         L.LbmHipBatchSetTuning.argtypes = [C.c_void_p, C.POINTER(LbmHipBatchTuning)]
         L.LbmHipBatchAutotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(LbmHipBatchTuning)]
         L.LbmHipBatchStats.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.LbmHipBatchGraphStats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                            C.POINTER(C.c_longlong)]
         L.LbmHipListRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
                                           C.POINTER(LbmHipSteady)]
         L.LbmHipRunSteady.argtypes = [KD, C.POINTER(CaseData), C.c_int, C.c_double, C.c_int,
@@ -553,6 +555,12 @@ class Batch:
         g, f = C.c_double(), C.c_double()
         self._check(hip_lib().LbmHipBatchStats(self.h, C.byref(g), C.byref(f)))
         return g.value, f.value
+
+    def graph_stats(self):
+        """(launch ranges of the last interval, graphs captured, graph replays)."""
+        n, c, r = C.c_int(), C.c_longlong(), C.c_longlong()
+        self._check(hip_lib().LbmHipBatchGraphStats(self.h, C.byref(n), C.byref(c), C.byref(r)))
+        return n.value, c.value, r.value
 
     def close(self):
         if self.h:

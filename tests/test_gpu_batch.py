@@ -1,6 +1,7 @@
 """GPU tests of the multi-lattice batch (LbmHipBatch*): many list-aa-pv-soa
 domains of different sizes, geometries, omegas and forces stepped together by
-one persistent launch per half-step, replayed from a HIP graph.
+one persistent launch per half-step (per half-step and group of domains),
+replayed from captured HIP graphs.
 
 Bar: every domain's node views after the batch equal, bit for bit, the same
 lattice run alone through the plugin's Kernel() for the same number of steps
@@ -96,13 +97,19 @@ def test_batch_membership_and_convergence(lbm, group_mb):
         c = b.add(*geos[1], max_steps=0)
         d = b.add(*geos[2], max_steps=100)
         assert b.step(20) == 3
+        n0, c0, r0 = b.graph_stats()               # one graph per group, captured and replayed
+        assert (c0, r0) == (n0, n0) and (n0 >= 2 if group_mb else n0 == 1)
         assert b.step(20) == 2                     # a reached its 40 steps
+        assert b.graph_stats() == (n0, c0, 2 * n0)  # same ranges: replayed, nothing captured
+        launched = 2 * n0
         assert (b.status(a).active, b.status(a).steps) == (0, 40)
         b.remove(c)                                # leaves mid-run
         e = b.add(*geos[3], max_steps=40)          # joins mid-run, reuses the free slot
         assert e == c
-        b.step(20)
-        b.step(20)
+        for _ in range(2):
+            b.step(20)
+            launched += b.graph_stats()[0]
+        assert b.graph_stats()[2] == launched
         st_d, st_e = b.status(d), b.status(e)
         assert (st_d.steps, st_e.steps, st_e.active) == (80, 40, 0)
         for i, (g, om, F), steps in ((a, geos[0], 40), (d, geos[2], 80), (e, geos[3], 40)):
@@ -186,11 +193,16 @@ def test_c4_batch_of_512_samples(lbm, oracle):
         with lbm.Batch(args + ["-prec", prec]) as b:
             ids = [b.add(g, s.omega, s.xforce, limit) for g, s in zip(geos, samples)]
             b.autotune(2)
-            active, rounds = len(ids), 0
+            active, rounds, launched = len(ids), 0, 0
             while active:
                 active = b.step(check, tol)
                 rounds += 1
+                launched += b.graph_stats()[0]
                 assert rounds <= limit // check + 1
+            # every interval replayed from captured graphs, one per group
+            slots, captures, replays = b.graph_stats()
+            print(f"c4 batch {prec}: {rounds} intervals, {replays} graph replays, {captures} captures", flush=True)
+            assert replays == launched and captures < replays
             st = [b.status(i) for i in ids]
             dumps = {j: b.dump(ids[j]) for j in stride + [small]}
         assert all(x.active == 0 for x in st)
