@@ -658,7 +658,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_list_odd (odd AA step, run-compressed adjacency)",
                      "scope": "frac is the dominant kernel's (the odd step); step_average_frac is the whole step's",
                      "achieved": round(odd_gbs, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic, "hbm_pair": pair,
+                     "frac": round(odd_gbs / PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic is not None else None,
+                     "hbm_pair": pair,
                      "bytes_per_flup": b_odd,
                      "bytes_basis": "list-AA odd balance: 19 loads + 19 stores + 18 int32 adjacency entries per "
                                     "node, adjacency compression not credited",
