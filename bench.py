@@ -194,11 +194,13 @@ def _batch_solve(lbm, W, samples, geos, prec, kargs, args):
                 _log(f"{prec}: {rounds * args.check} steps, {active} active")
         wall = time.perf_counter() - t0
         gpu_s, flups = b.stats()
+        _, captures, replays = b.graph_stats()
         st = [b.status(i) for i in ids]
     perm = [W.permeability(x.momentumX, x.nFluid, s.cells, s.omega, s.xforce) for x, s in zip(st, samples)]
     rec = {"solves_per_s": round(len(ids) / (setup + wall), 3), "wall_s": round(setup + wall, 3),
            "run_s": round(wall, 3), "gpu_s": round(gpu_s, 3), "setup_s": round(setup, 2),
-           "mflups": round(flups / wall / 1e6, 1), "converged": sum(x.converged for x in st),
+           "mflups": round(flups / wall / 1e6, 1), "intervals": rounds,
+           "graph_replays": replays, "graph_captures": captures, "converged": sum(x.converged for x in st),
            "steps_median": W.percentile([x.steps for x in st], 50), "steps_max": max(x.steps for x in st),
            "tuning": {"even_block": tune.evenBlock, "even_cap": tune.evenCap, "vec": tune.vec,
                       "odd_block": tune.oddBlock, "even_ms": round(tune.evenMs, 4), "odd_ms": round(tune.oddMs, 4),
