@@ -368,51 +368,30 @@ def precision_mode(args, lbm):
                    {"higher_is_better": False, "precision": out})
 
 
-def _plain_service(lbm, W, samples, kargs, interval, tol, max_steps):
-    """The plain path: each request alone through geometry, Init, initial
-    state, LbmHipRunSteady and Deinit in FP64; service times, FP64
-    permeabilities and node updates."""
-    service, perm64, flups = [], [], 0.0
-    for s, _ in samples:
-        t0 = time.perf_counter()
-        geo = lbm.Geometry.from_string(s.geometry, s.dims)
-        with lbm.Kernel("list-aa-pv-soa", geo, kargs + ["-quiet"]) as k:
-            k.set_initial_weights()
-            k.set_shadow_download(False)
-            r = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), interval, tol, max_steps)
-        service.append(time.perf_counter() - t0)
-        perm64.append(W.permeability(r.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce))
-        flups += geo.n_fluid * r.steps
-    _log(f"plain fp64 path: {len(samples)} requests, {sum(service):.1f} s of service")
-    return service, perm64, flups
-
-
 def serving_record(lbm, n=16, rate=500.0, seed=2025, tol=1e-6, interval=100, max_steps=20000, budget=200e9):
     """The serving half of the metric on a short stream of the configured
     shape: n Poisson arrivals at `rate`, porous cubes of edge U[32,192],
     FP64 or shifted-FP32 tag, through the continuous-batching server under
-    `budget` bytes; against the plain path (each request alone in FP64,
-    queued in arrival order: measured service times, Lindley's recursion)."""
+    `budget` bytes; against the plain path run on the same arrival clock
+    (each request alone in FP64, in arrival order)."""
     import serving as S
-    import workloads as W
 
     samples = S.serving_samples(n, seed)
     kargs = ["-lambda", repr(LAMBDA)]
-    service, perm64, flups_plain = _plain_service(lbm, W, samples, kargs, interval, tol, max_steps)
     arr = S.poisson_arrivals(n, rate, seed + int(rate))
     reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arr, samples))]
+    solver = S.PlainGpuSolver(lbm, kargs, interval, tol, max_steps)
+    plain_recs = S.serve_plain(reqs, solver)
+    plain = S.summarize(plain_recs, solver.node_updates)
+    perm64 = {r.rid: r.permeability for r in plain_recs}
+    _log(f"plain fp64 path: {plain}")
     be = S.GpuBackend(lbm, kargs, max_steps)
     try:
         recs = S.serve(reqs, be, int(budget), interval, tol)
         served = S.summarize(recs, be.flups())
     finally:
         be.close()
-    fin = S.fifo_latencies(arr, service)
-    lat = [f - a for f, a in zip(fin, arr)]
-    span = fin[-1] - arr[0]
-    plain = {"p50_latency_s": round(W.percentile(lat, 50), 4), "p99_latency_s": round(W.percentile(lat, 99), 4),
-             "solves_per_s": round(n / span, 3), "mflups": round(flups_plain / span / 1e6, 1)}
-    err = [abs(r.permeability - p) / abs(p) for r, p in zip(recs, perm64) if r.prec == "fp32s" and not r.rejected]
+    err = [abs(r.permeability - perm64[r.rid]) / abs(perm64[r.rid]) for r in recs if r.prec == "fp32s" and not r.rejected]
     served = {k: (float(v) if hasattr(v, "item") else v) for k, v in served.items()}
     served["permeability_rel_err_fp32s_vs_fp64_max"] = max(err) if err else None
     # the same stream with every request tagged shifted FP32
@@ -424,12 +403,14 @@ def serving_record(lbm, n=16, rate=500.0, seed=2025, tol=1e-6, interval=100, max
     finally:
         be.close()
     served32 = {k: (float(v) if hasattr(v, "item") else v) for k, v in served32.items()}
-    served32["permeability_rel_err_vs_fp64_max"] = max(abs(r.permeability - p) / abs(p) for r, p in zip(recs32, perm64))
-    plain = {k: float(v) for k, v in plain.items()}
+    served32["permeability_rel_err_vs_fp64_max"] = max(abs(r.permeability - perm64[r.rid]) / abs(perm64[r.rid])
+                                                       for r in recs32)
+    plain = {k: (float(v) if hasattr(v, "item") else v) for k, v in plain.items()}
     _log(f"serving: mixed {served} | all fp32s {served32} | plain {plain}")
     return {"workload": f"{n} requests, Poisson {rate:g}/s (seed {seed}), porous cubes of edge U[32,192], "
                         f"porosity U[0.5,0.9], radius U[3,8], omega U[0.8,1.6], F 1e-6, FP64 or shifted-FP32 tag, "
-                        f"to relative x-momentum change < {tol:g} per {interval} steps, {budget / 1e9:g} GB budget",
+                        f"to relative x-momentum change < {tol:g} per {interval} steps, {budget / 1e9:g} GB budget; "
+                        f"plain path run on the same arrival clock, one request at a time in FP64",
             "served": served, "served_all_fp32s": served32, "plain_fp64": plain,
             "solves_speedup": round(served["solves_per_s"] / plain["solves_per_s"], 3),
             "solves_speedup_all_fp32s": round(served32["solves_per_s"] / plain["solves_per_s"], 3)}
@@ -439,31 +420,29 @@ def serve_mode(args, lbm):
     """Poisson streams of permeability requests (porous cubes, edge U[32,192],
     FP64 or shifted-FP32 tag) through the continuous-batching server, one
     stream per rate; the plain path solves the same requests one at a time in
-    FP64 (service times measured once, queueing by Lindley's recursion)."""
+    FP64 on the same arrival clock."""
     import serving as S
-    import workloads as W
 
     rates = [float(r) for r in args.rates.split(",")]
     samples = S.serving_samples(args.requests, args.seed)
     kargs = ["-lambda", repr(LAMBDA), "-pad", "16384+128"]
     max_steps, tol, interval = 20000, args.tol, args.check
-    service, perm64, flups_plain = _plain_service(lbm, W, samples, kargs, interval, tol, max_steps)
-    out = {}
+    out, perm64 = {}, {}
     for rate in rates:
         arr = S.poisson_arrivals(len(samples), rate, args.seed + int(rate))
         reqs = [S.Request(i, a, s, p) for i, (a, (s, p)) in enumerate(zip(arr, samples))]
+        solver = S.PlainGpuSolver(lbm, kargs, interval, tol, max_steps)
+        plain_recs = S.serve_plain(reqs, solver)
+        plain = S.summarize(plain_recs, solver.node_updates)
+        perm64 = {r.rid: r.permeability for r in plain_recs}
         be = S.GpuBackend(lbm, kargs, max_steps)
         try:
             recs = S.serve(reqs, be, int(args.budget_gb * 1e9), interval, tol)
             summ = S.summarize(recs, be.flups())
         finally:
             be.close()
-        fin = S.fifo_latencies(arr, service)
-        lat = [f - a for f, a in zip(fin, arr)]
-        span = fin[-1] - arr[0]
-        plain = {"p50_latency_s": round(W.percentile(lat, 50), 4), "p99_latency_s": round(W.percentile(lat, 99), 4),
-                 "solves_per_s": round(len(arr) / span, 3), "mflups": round(flups_plain / span / 1e6, 1)}
-        err = [abs(r.permeability - p) / abs(p) for r, p in zip(recs, perm64) if r.prec == "fp32s" and not r.rejected]
+        err = [abs(r.permeability - perm64[r.rid]) / abs(perm64[r.rid]) for r in recs
+               if r.prec == "fp32s" and not r.rejected]
         summ["permeability_rel_err_fp32s_vs_fp64_max"] = max(err) if err else None
         summ["fp32s_requests"] = len(err)
         out[str(int(rate))] = {"served": summ, "plain_fp64": plain,
@@ -479,7 +458,7 @@ def serve_mode(args, lbm):
         summ = S.summarize(recs, be.flups())
     finally:
         be.close()
-    err = [abs(r.permeability - p) / abs(p) for r, p in zip(recs, perm64)]
+    err = [abs(r.permeability - perm64[r.rid]) / abs(perm64[r.rid]) for r in recs]
     summ["permeability_rel_err_vs_fp64_max"] = max(err)
     plain = out[str(int(rate))]["plain_fp64"]
     out["fp32s_only_at_top_rate"] = {"served": summ, "solves_speedup": round(summ["solves_per_s"] / plain["solves_per_s"], 2)}

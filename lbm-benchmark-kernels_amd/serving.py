@@ -224,9 +224,52 @@ class GpuBackend:
             b.close()
 
 
+class PlainGpuSolver:
+    """The plain path for one request: its lattice alone in FP64 -- geometry,
+    Init (list build), f = w, LbmHipRunSteady in checks of `interval` steps
+    to `tol` or `max_steps`, Deinit."""
+
+    def __init__(self, lbm, kernel_args=(), interval=100, tol=1e-6, max_steps=20000):
+        self.lbm = lbm
+        self.kernel_args = list(kernel_args)
+        self.interval, self.tol, self.max_steps = interval, tol, max_steps
+        self.node_updates = 0.0
+
+    def __call__(self, r: Request):
+        lbm, s = self.lbm, r.sample
+        geo = lbm.Geometry.from_string(s.geometry, s.dims)
+        with lbm.Kernel("list-aa-pv-soa", geo, self.kernel_args + ["-prec", "fp64", "-quiet"]) as k:
+            k.set_initial_weights()
+            k.set_shadow_download(False)
+            res = k.run_steady(lbm.case_data(omega=s.omega, xforce=s.xforce, iterations=0), self.interval, self.tol,
+                               self.max_steps)
+        self.node_updates += geo.n_fluid * res.steps
+        return res.steps, res.converged, W.permeability(res.momentumX, geo.n_fluid, s.cells, s.omega, s.xforce), \
+            geo.n_fluid
+
+
+def serve_plain(requests, solve, clock=time.perf_counter, sleep=time.sleep):
+    """The plain one-request-at-a-time path, run on the arrival clock: the
+    requests in arrival order, each solved alone by ``solve(request) ->
+    (steps, converged, permeability, n_fluid)`` once it has arrived and the
+    one before it is answered; every request is computed in FP64 whatever
+    its tag."""
+    t0 = clock()
+    recs = []
+    for r in sorted(requests, key=lambda q: q.arrival):
+        now = clock() - t0
+        if now < r.arrival:
+            sleep(r.arrival - now)
+        rec = Record(r.rid, "fp64", r.arrival, admitted=clock() - t0)
+        rec.steps, rec.converged, rec.permeability, rec.n_fluid = solve(r)
+        rec.finished = clock() - t0
+        recs.append(rec)
+    return recs
+
+
 def fifo_latencies(arrivals, service):
     """Finish times of a single server taking requests in arrival order
-    (Lindley's recursion); the plain one-request-at-a-time path."""
+    (Lindley's recursion): what serve_plain gives for these service times."""
     finish, t = [], -math.inf
     for a, s in zip(arrivals, service):
         t = max(t, a) + s

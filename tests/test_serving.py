@@ -152,6 +152,30 @@ def test_fifo_latencies_and_summary():
     assert s["makespan_s"] == 5.5 and s["p50_latency_s"] == pytest.approx(W.percentile([1, 1.5, 3.4, 0.5], 50))
 
 
+def test_plain_path_on_the_arrival_clock():
+    """serve_plain on a simulated clock: requests taken in arrival order,
+    each once it has arrived and the one before it is answered -- the finish
+    times of Lindley's recursion over the same service times; every request
+    computed in FP64."""
+    arr = [0.0, 0.5, 0.6, 5.0, 5.1]
+    service = [1.0, 1.0, 2.0, 0.5, 0.25]
+    clk = Clock()
+    sample = W.Sample(0, (8, 8, 8), 2.0, 0.8, 1, 1.0, 1e-6)
+    reqs = [S.Request(i, a, sample, "fp32s" if i % 2 else "fp64") for i, a in enumerate(arr)]
+
+    def solve(r):
+        assert clk() >= r.arrival
+        clk.t += service[r.rid]
+        return 100 + r.rid, 1, 0.5, 10
+
+    recs = S.serve_plain(list(reversed(reqs)), solve, clock=clk, sleep=clk.sleep)
+    assert [r.rid for r in recs] == list(range(len(arr)))
+    assert [r.finished for r in recs] == pytest.approx(S.fifo_latencies(arr, service))
+    assert all(r.prec == "fp64" and r.arrival <= r.admitted < r.finished for r in recs)
+    assert [r.steps for r in recs] == [100 + i for i in range(len(arr))]
+    assert S.summarize(recs)["p50_latency_s"] == pytest.approx(W.percentile([1.0, 1.5, 3.4, 0.5, 0.65], 50))
+
+
 def test_generators():
     a = W.batch_samples(200, 7)
     assert a == W.batch_samples(200, 7)
