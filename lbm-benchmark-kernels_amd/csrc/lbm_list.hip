@@ -1246,8 +1246,14 @@ extern "C" int LbmHipListDeviceBytes(const LatticeDesc * ld, const Parameters * 
 	}
 	const uint64_t N = (uint64_t)ld->Dims[0] * ld->Dims[1] * ld->Dims[2];
 	const uint64_t nF = (uint64_t)ld->nFluid;
+	// what Init refuses, refused here too
+	if (19ull * N > 0xffffffffull) { fail("number of PDFs exceed 2^32 or empty dimensions."); return -1; }
 	const long nCells = pad_cells(ld->nFluid, (int)sizeof(PdfT), a.havePad ? &a.pad : nullptr);
 	if (nCells < 0) { fail("LbmHipListDeviceBytes: padding failed."); return -1; }
+	if (19ull * (uint64_t)nCells > 0xffffffffull) {
+		fail("padded node count %ld does not fit the 32-bit adjacency.", nCells);
+		return -1;
+	}
 	const uint64_t adjStride = (nF + 255) / 256 * 256, nWaves = (nF + 63) / 64;
 	const uint64_t nBlocks = (N + kNumBlock - 1) / kNumBlock, oddBlocks = (nF + a.blockSize - 1) / a.blockSize;
 	const uint64_t sizes[] = {

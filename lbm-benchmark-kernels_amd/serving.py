@@ -51,6 +51,7 @@ class Record:
     permeability: float = math.nan
     bytes: int = 0
     rejected: bool = False
+    error: str = ""             # why the library refused the lattice, if it did
     nodes: np.ndarray | None = field(default=None, repr=False)
 
     @property
@@ -94,7 +95,9 @@ def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.
     """Runs the request stream to completion; returns one Record per request.
 
     A request's device bytes come from backend.size(request) when the
-    backend has it, else from device_bytes.  Waiting requests are admitted
+    backend has it, else from device_bytes; a request whose lattice the
+    library refuses to size (RuntimeError: more PDFs than its 32-bit indices
+    address) is rejected with the library's message.  Waiting requests are admitted
     first fit in arrival order; a request that does not fit is passed over
     by later ones at most max_bypass times (0: strict arrival order), after
     which nothing behind it is admitted until it fits."""
@@ -119,13 +122,17 @@ def serve(requests, backend, budget: int, interval: int, tol: float, clock=time.
         while i < len(queue):
             r = queue[i]
             if r.rid not in need_of:
-                need_of[r.rid] = size(r) if size else device_bytes(r.sample, r.prec)
+                try:
+                    need_of[r.rid] = size(r) if size else device_bytes(r.sample, r.prec)
+                except RuntimeError as e:
+                    need_of[r.rid] = None
+                    recs[r.rid].error = str(e)
                 passed[r.rid] = 0
             need = need_of[r.rid]
-            if need > budget:
+            if need is None or need > budget:
                 queue.pop(i)
                 rec = recs[r.rid]
-                rec.rejected, rec.bytes, rec.finished = True, need, clock() - t0
+                rec.rejected, rec.bytes, rec.finished = True, need or 0, clock() - t0
                 continue
             if in_use + need > budget:
                 if passed[r.rid] >= max_bypass:

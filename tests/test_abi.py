@@ -172,6 +172,25 @@ def test_aa_rejects_grid_extents_over_limit(lbm):
         lbm.Kernel("aa-soa", geo)
 
 
+def test_list_build_no_alloc_refuses_oversized_boxes(lbm):
+    """LbmHipListBuildNoAlloc checks the box before it touches the device: a
+    box of more than 2^32 PDFs, or a node count whose adjacency does not fit
+    32 bits, is refused with the reference's message and *nFluid = 0.  The
+    pointers are placeholders: the checks come first, nothing is read."""
+    L = lbm.hip_lib()
+    L.LbmHipSetErrorMode(lbm.LBM_HIP_ERRORS_RETURN)
+    I3 = C.c_int * 3
+    n = C.c_int(-1)
+    fake = C.c_void_p(4096)
+    for dims, n_cells, msg in (((1024, 1024, 256), 1000, "exceed 2^32"),
+                               ((1000, 1000, 226), 226_492_416, "out of range")):
+        L.LbmHipClearError()
+        rc = L.LbmHipListBuildNoAlloc(fake, I3(*dims), I3(1, 0, 0), None, n_cells, 1024, 1000, fake, fake, fake,
+                                      C.byref(n))
+        assert rc == -1 and n.value == 0, dims
+        assert msg in L.LbmHipLastError().decode(), dims
+
+
 def test_batch_fails_loudly_without_device(lbm):
     if not _no_gpu(lbm):
         pytest.skip("a GPU is present")

@@ -144,6 +144,27 @@ def test_oversized_request_is_rejected():
     assert sum(not r.rejected for r in recs) >= 1
 
 
+def test_request_the_library_refuses_is_rejected():
+    """A backend whose size() raises (the library refusing a lattice too
+    large for its 32-bit indices) gets that request rejected with the
+    message; the others are served."""
+    reqs = _requests(6, 50)
+
+    class Refusing(SimBackend):
+        def size(self, r):
+            if r.rid == 2:
+                raise RuntimeError("number of PDFs exceed 2^32 or empty dimensions.")
+            return S.device_bytes(r.sample, r.prec)
+
+    clk = Clock()
+    be = Refusing(clk)
+    recs = S.serve(reqs, be, 1 << 50, 100, 1e-6, clock=clk, sleep=clk.sleep)
+    assert [r.rejected for r in recs] == [r.rid == 2 for r in reqs]
+    assert "exceed 2^32" in recs[2].error and recs[2].bytes == 0
+    assert 2 not in be.admit_order and len(be.admit_order) == 5
+    assert all(r.steps > 0 for r in recs if not r.rejected)
+
+
 def test_fifo_latencies_and_summary():
     fin = S.fifo_latencies([0.0, 0.5, 0.6, 5.0], [1.0, 1.0, 2.0, 0.5])
     assert fin == [1.0, 2.0, 4.0, 5.5]
@@ -206,6 +227,26 @@ def test_permeability_of_plane_poiseuille(oracle):
     k_exact = 0.5 * float((zp * (h - zp)).sum()) / nz
     k = W.permeability(jx, d.shape[0], nx * ny * nz, omega, F)
     assert k == pytest.approx(k_exact, rel=1e-6)
+
+
+def test_device_bytes_refuses_what_init_refuses(lbm):
+    """The size bound refuses the lattices Init refuses (host only, no mask
+    read): more than 2^32 PDFs in the box, or a padded node count whose
+    adjacency values do not fit 32 bits -- so the serving loop rejects such
+    a request instead of admitting it and failing at Init."""
+    import types
+    ld = lbm.LatticeDesc()
+    ld.Dims[0], ld.Dims[1], ld.Dims[2] = 1024, 1024, 256
+    ld.nFluid = 1000
+    with pytest.raises(RuntimeError, match=r"exceed 2\^32"):
+        lbm.list_device_bytes(types.SimpleNamespace(ld=ld))
+    # 226,000,000 all-fluid cells: 19 x 226e6 < 2^32, but padded to whole
+    # 16 MiB planes (2,097,152 cells) the node count no longer fits
+    ld.Dims[0], ld.Dims[1], ld.Dims[2] = 1000, 1000, 226
+    ld.nFluid = 226_000_000
+    assert lbm.list_device_bytes(types.SimpleNamespace(ld=ld), ["-pad", "no"]) > 19 * 8 * ld.nFluid
+    with pytest.raises(RuntimeError, match="does not fit the 32-bit adjacency"):
+        lbm.list_device_bytes(types.SimpleNamespace(ld=ld), ["-pad", "16777216+0"])
 
 
 def test_device_bytes_bound_covers_the_lattice(lbm, oracle):
