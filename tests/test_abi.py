@@ -172,6 +172,19 @@ def test_aa_rejects_grid_extents_over_limit(lbm):
         lbm.Kernel("aa-soa", geo)
 
 
+@pytest.mark.gpu
+def test_init_refuses_more_than_2_32_pdfs(lbm):
+    """Both kernels refuse a box of more than 2^32 PDFs at Init, before any
+    allocation (the reference's check, src/Main.c:399-406): 1024 x 1024 x 217
+    all-fluid cells, 19 x 227.5e6 PDFs."""
+    dims = (1024, 1024, 217)
+    geo = lbm.Geometry.from_mask(np.ones(int(np.prod(dims)), dtype=np.int32), dims, (1, 0, 0))
+    for name in ("list-aa-pv-soa", "aa-soa"):
+        with pytest.raises(RuntimeError, match=r"exceed 2\^32"):
+            lbm.Kernel(name, geo)
+    del geo
+
+
 def test_list_build_no_alloc_refuses_oversized_boxes(lbm):
     """LbmHipListBuildNoAlloc checks the box before it touches the device: a
     box of more than 2^32 PDFs, or a node count whose adjacency does not fit
