@@ -57,7 +57,8 @@ def tiny_cases():
 
 
 def blk_args(g):
-    """Kernel arguments stored in a tiny fixture (node-numbering blocks)."""
+    """Kernel arguments stored in a tiny fixture (node-numbering blocks,
+    padding)."""
     return [str(a) for a in g["kernel_args"] if str(a)]
 
 
@@ -66,9 +67,19 @@ def blk_tuple(args):
     blk = [0, 0, 0]
     it = iter(args)
     for a in it:
-        v = int(next(it))
+        v = next(it)
         if a == "-blk":
-            blk = [v, v, v]
-        else:
-            blk["xyz".index(a[-1])] = v
+            blk = [int(v)] * 3
+        elif a.startswith("-blk-"):
+            blk["xyz".index(a[-1])] = int(v)
     return tuple(blk)
+
+
+def pad_arg(args):
+    """The -pad string among the words, or None (the reference's auto)."""
+    it = iter(args)
+    for a in it:
+        v = next(it)
+        if a == "-pad":
+            return v
+    return None

@@ -28,7 +28,7 @@ and outputs:
                     both kernels' deviation from the Poiseuille profile,
                     SHA-256 of the node views, every 97th node
 
-Usage: python tests/golden/gen_golden.py [--skip-large] [--only NAME]
+Usage: python tests/golden/gen_golden.py [--skip-large] [--only NAME] [--tiny NAME ...]
 Needs oracle/_ref (built from the reference sources) and liblbmhost.so
 (for the porous generator).  The reference is never imported here as
 Python; it is compiled C called through ctypes.
@@ -63,6 +63,9 @@ TINY = [
     ("pipe12x10x10", "pipe", (12, 10, 10), (1, 0, 0), ()),
     ("porous16", "porous-2-0.7-7", (16, 14, 12), (0, 0, 0), ()),
     ("fluid6", "fluid", (6, 5, 4), (0, 0, 0), ()),
+    # the reference's -pad strings: node counts and adjacency values follow them
+    ("porous16_pad", "porous-2-0.7-7", (16, 14, 12), (0, 0, 0), ("-pad", "64+8,1024+128")),
+    ("channel10x9x7_padno_blk3", "channel", (10, 9, 7), (1, 0, 0), ("-pad", "no", "-blk", "3")),
 ]
 
 
@@ -128,10 +131,31 @@ def verify64(ref, manifest):
     print("verify64_12000", rl.verifyNorm, ra.verifyNorm, time.time() - t0, flush=True)
 
 
+def tiny(ref, manifest, name, gstr, dims, per, kargs):
+    """A tiny case: full node dumps of the reference's list kernel (1 and 3
+    threads) and aa-soa after 100 steps, with its list build."""
+    mask, ldp, nf = geometry(gstr, dims, per)
+    steps, omega, force = 100, 1.3, 1e-5
+    dl1, b, r1 = ref.list_run(mask, dims, ldp, steps, omega, force, n_threads=1, kernel_args=kargs)
+    dl3, _, r3 = ref.list_run(mask, dims, ldp, steps, omega, force, n_threads=3, kernel_args=kargs)
+    da, ra = ref.aa_run(mask, dims, ldp, steps, omega, force, n_threads=2)
+    np.savez_compressed(os.path.join(OUT, f"tiny_{name}.npz"),
+                        mask=mask.astype(np.int8), dims=np.array(dims), periodic=np.array(ldp),
+                        kernel_args=np.array(list(kargs) or [""]), steps=steps, omega=omega, xforce=force,
+                        list_t1=dl1, list_t3=dl3, aa_t2=da, adj=b.adj, grid=b.grid, coords=b.coords,
+                        n_cells=r1.nCells, n_bb=ra.nBounceBack, runs_t1=r1.nConsecNodes,
+                        runs_t3=r3.nConsecNodes)
+    manifest[f"tiny_{name}"] = {"geometry": gstr, "dims": dims, "periodic": ldp, "nFluid": nf,
+                                "nCells": r1.nCells}
+    print("tiny", name, nf, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
     ap.add_argument("--only", choices=["porous256_500", "verify64_12000"])
+    ap.add_argument("--tiny", action="append", choices=[t[0] for t in TINY],
+                    help="write only this tiny case (repeatable)")
     args = ap.parse_args()
     ref = Reference("verif")
     with open(os.path.join(OUT, "manifest.json")) as f:
@@ -144,21 +168,14 @@ def main():
             {"porous256_500": porous256_500, "verify64_12000": verify64}[args.only](ref, manifest)
             write_manifest(manifest)
             return
-        for name, gstr, dims, per, kargs in TINY:
-            mask, ldp, nf = geometry(gstr, dims, per)
-            steps, omega, force = 100, 1.3, 1e-5
-            dl1, b, r1 = ref.list_run(mask, dims, ldp, steps, omega, force, n_threads=1, kernel_args=kargs)
-            dl3, _, r3 = ref.list_run(mask, dims, ldp, steps, omega, force, n_threads=3, kernel_args=kargs)
-            da, ra = ref.aa_run(mask, dims, ldp, steps, omega, force, n_threads=2)
-            np.savez_compressed(os.path.join(OUT, f"tiny_{name}.npz"),
-                                mask=mask.astype(np.int8), dims=np.array(dims), periodic=np.array(ldp),
-                                kernel_args=np.array(list(kargs) or [""]), steps=steps, omega=omega, xforce=force,
-                                list_t1=dl1, list_t3=dl3, aa_t2=da, adj=b.adj, grid=b.grid, coords=b.coords,
-                                n_cells=r1.nCells, n_bb=ra.nBounceBack, runs_t1=r1.nConsecNodes,
-                                runs_t3=r3.nConsecNodes)
-            manifest[f"tiny_{name}"] = {"geometry": gstr, "dims": dims, "periodic": ldp, "nFluid": nf,
-                                        "nCells": r1.nCells}
-            print("tiny", name, nf, flush=True)
+        if args.tiny:
+            for entry in TINY:
+                if entry[0] in args.tiny:
+                    tiny(ref, manifest, *entry)
+            write_manifest(manifest)
+            return
+        for entry in TINY:
+            tiny(ref, manifest, *entry)
 
         # the reference's -verify preset (src/Main.c:232-265)
         dims = (16, 16, 16)

@@ -7,7 +7,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import golden, tiny_cases
+from conftest import blk_tuple, golden, pad_arg, tiny_cases
 
 
 def _sha(a):
@@ -20,17 +20,11 @@ def test_tiny_case(oracle, case):
     dims, per = tuple(g["dims"]), tuple(g["periodic"])
     mask = g["mask"].astype(np.int32)
     kargs = [str(a) for a in g["kernel_args"] if str(a)]
-    blk = [0, 0, 0]
-    it = iter(kargs)
-    for a in it:
-        v = int(next(it))
-        if a == "-blk":
-            blk = [v, v, v]
-        else:
-            blk["xyz".index(a[-1])] = v
     steps, omega, F = int(g["steps"]), float(g["omega"]), float(g["xforce"])
 
-    b = oracle.list_build(mask, dims, per, tuple(blk))
+    pad = pad_arg(kargs)
+    n_cells = oracle.pad_cells(int(np.count_nonzero(mask)), pad) if pad else None
+    b = oracle.list_build(mask, dims, per, blk_tuple(kargs), n_cells=n_cells)
     assert b.n_cells == int(g["n_cells"])
     assert np.array_equal(b.adj, g["adj"])
     assert np.array_equal(b.coords, g["coords"])
