@@ -72,9 +72,13 @@
  *                     nodes (aa-soa: consecutive z cells) per thread with
  *                     8-byte accesses (auto, default; the list kernel only
  *                     when the padded node count allows), or one per
- *                     thread (1); at fp32s aa-soa's odd step takes the
- *                     same pairs of z cells, with one access per direction
- *                     for a pair of interior fluid cells
+ *                     thread (1)
+ *   -odd-pairs auto|on|off
+ *                     aa-soa at fp32s with -vec auto: the odd step takes
+ *                     pairs of z cells, one access per direction for a pair
+ *                     of interior fluid cells (auto, default: on lattices
+ *                     over 256 MiB, where it measured faster; one cell per
+ *                     thread on smaller ones)
  */
 #ifndef LBM_HIP_H
 #define LBM_HIP_H

@@ -555,10 +555,11 @@ __global__ __launch_bounds__(256) void k_aa_moments(const typename Codec::S * __
 // the two launches (per-launch timing) or null.
 struct AaLaunch {
 	dim3 grid, block;        // odd step: one z row segment per block
-	dim3 vgrid, vblock;      // odd step over groups of vec cells (FP32)
+	dim3 vgrid, vblock;      // odd step over pairs of z cells (FP32)
 	dim3 egrid, eblock;      // even step: blockDim.y rows of y per block
 	size_t elds;
-	int vec;                 // cells per thread in both steps (1, or VMAX of the codec)
+	int vec;                 // cells per thread in the even step (1, or VMAX of the codec)
+	bool oddPairs;           // FP32 odd step over pairs of z cells
 	hipStream_t stream;
 	const uint32_t * info;               // odd step
 	const unsigned long long * fluid;    // even step
@@ -607,7 +608,7 @@ static void aa_launch(const AaLaunch & L, typename Codec::S * pdf, Codec cod, co
 	// 2-byte alignment) measured 1.5x slower than one cell per thread
 	constexpr int V = Codec::VMAX;
 	if constexpr (V > 1 && sizeof(typename Codec::S) == 4) {
-		if (L.vec == V) {
+		if (L.oddPairs) {
 			if (fs)          k_aa_odd_v<Codec, V, true, true><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 			else if (force)  k_aa_odd_v<Codec, V, true, false><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
 			else             k_aa_odd_v<Codec, V, false, false><<<L.vgrid, L.vblock, 0, L.stream>>>(pdf, cod, L.info, L.g, k, ft);
@@ -653,19 +654,21 @@ static bool aa_output_sync(AaData * A)
 
 // Threads along z of a block over rows of n cells (or cell groups), at
 // most 256: the odd step splits a row into equal parts rounded up to a
-// wavefront (384 cells: 2 x 192); the even step, whose blocks take
-// 512 / threads rows, takes the power of two from 64 to 256 that leaves
-// the fewest idle threads in a row's last block (384 cells: 3 x 128).
+// wavefront (384 cells: 2 x 192); the even step (and the FP32 odd step over
+// pairs), whose blocks take several rows, takes the power of two from
+// `least` to 256 that leaves the fewest idle threads in a row's last block
+// (384 cells: 3 x 128; 32 FP32 pairs of a 64-cell row: 32, two rows per
+// wavefront).  `least` keeps a row segment at 128 B or more.
 static int odd_row_threads(int n)
 {
 	const int parts = (n + 255) / 256;
 	return ((n + parts - 1) / parts + 63) / 64 * 64;
 }
 
-static int even_row_threads(int n)
+static int even_row_threads(int n, int least)
 {
 	int best = 256, idle = 1 << 30;
-	for (int b = 256; b >= 64; b /= 2) {
+	for (int b = 256; b >= least; b /= 2) {
 		const int i = (n + b - 1) / b * b - n;
 		if (i < idle) { best = b; idle = i; }
 	}
@@ -721,12 +724,13 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const AaGeom g = A->g;
 	const int bs = odd_row_threads(g.Z);
 	const dim3 block((unsigned)bs), grid((unsigned)((g.Z + bs - 1) / bs), (unsigned)g.Y, (unsigned)g.X);
-	// V cells per thread for the narrow types (-vec auto; the odd step at
-	// FP32 only); even step blocks of 512 threads over several y rows at
-	// one block per CU, as the list kernel's even step (stream_shape)
+	// V cells per thread for the narrow types (-vec auto); even step blocks
+	// of 512 threads over several y rows at one block per CU, as the list
+	// kernel's even step (stream_shape)
 	const int vec = c.args.evenVec == 0 ? 1 : c.args.prec == PREC_FP32S ? 2 : c.args.prec == PREC_FP16S ? 4 : 1;
 	const int zq = (g.Z + vec - 1) / vec;
-	const int ebs = even_row_threads(zq);
+	const int laneBytes = (int)elem_bytes(c.args.prec) * vec;
+	const int ebs = even_row_threads(zq, laneBytes >= 8 ? 16 : 128 / laneBytes);
 	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
 	// Defaults measured at 64^3 and 256^3 (duct channel): on lattices larger
 	// than the 256 MiB last-level cache non-temporal accesses (FP64 with the
@@ -736,13 +740,20 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const bool big = aa_bytes(A) > (size_t)256 << 20;
 	const StreamShape es = (big && c.args.prec != PREC_FP64 && !c.args.evenCapGiven)
 	                       ? StreamShape{ebs, 0} : stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
-	const unsigned rows = (unsigned)(es.block / ebs);
+	// at least one wavefront per block when rows are shorter than one
+	const unsigned rows = (unsigned)(es.block / ebs) * (unsigned)ebs < 64u ? 64u / (unsigned)ebs
+	                                                                     : (unsigned)(es.block / ebs);
 	const dim3 eblock((unsigned)ebs, rows), egrid(egx, ((unsigned)g.Y + rows - 1) / rows, grid.z);
 	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && big);
-	// odd step over groups of vec cells: blocks of 256 threads over 256 / ebs rows of y
+	// FP32 odd step over pairs of z cells: blocks of 256 threads over 256 / ebs
+	// rows of y; measured faster than one cell per thread only on lattices
+	// over 256 MiB (256^3 duct 0.488 against 0.497 ms; 64^3 27.7 against
+	// 15.2 us, 128^3 69.6 against 61.0 us)
+	const bool oddPairs = vec == 2 && (c.args.oddPairs == 1 || (c.args.oddPairs == -1 && big));
 	const unsigned vrows = (unsigned)(256 / ebs);
 	const dim3 vblock((unsigned)ebs, vrows), vgrid(egx, ((unsigned)g.Y + vrows - 1) / vrows, (unsigned)g.X);
-	const AaLaunch L = {grid, block, vgrid, vblock, egrid, eblock, es.lds, vec, c.stream, A->dInfo, A->dFluid, g, nt};
+	const AaLaunch L = {grid, block, vgrid, vblock, egrid, eblock, es.lds, vec, oddPairs, c.stream, A->dInfo,
+	                    A->dFluid, g, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto half = [&](bool on, bool fl, bool fs, int it, bool doEven, bool doOdd) {
 		hipEvent_t * ev = perLaunch ? c.ev + 2 * it : nullptr;

@@ -46,7 +46,9 @@ struct KernelArgs {
 	int evenCapGiven = 0;  // -even-cap was on the command line (else the kernel's default applies)
 	int prec = 0;          // list: PDF storage, see enum Prec
 	int f16Shift = 12;     // FP16 storage holds (f - w) * 2^f16Shift
-	int evenVec = 1;       // list, narrow storage: several nodes per thread in the even step
+	int evenVec = 1;       // narrow storage: several nodes per thread in the even step
+	int oddPairs = -1;     // aa-soa, FP32 storage: odd step over pairs of z cells (1 on, 0 off,
+	                       // -1 on lattices over 256 MiB, where it measured faster)
 	int quiet = 0;         // no informational lines at Init
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)

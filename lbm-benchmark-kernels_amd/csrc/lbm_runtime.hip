@@ -188,6 +188,13 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (!strcmp(v, "auto")) a->evenVec = 1;
 			else if (!strcmp(v, "1")) a->evenVec = 0;
 			else return fail("argument -vec requires auto or 1.");
+		} else if (!listKernel && (is("-odd-pairs") || is("--odd-pairs"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -odd-pairs requires auto, on or off.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "auto")) a->oddPairs = -1;
+			else if (!strcmp(v, "on")) a->oddPairs = 1;
+			else if (!strcmp(v, "off")) a->oddPairs = 0;
+			else return fail("argument -odd-pairs requires auto, on or off.");
 		} else if (listKernel && (is("-shadow") || is("--shadow"))) {
 			if (i + 1 >= p->nKernelArgs) return fail("argument -shadow requires eager or lazy.");
 			const char * v = p->KernelArgs[++i];
@@ -234,7 +241,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-statistics] [-graph on|off]\n");
 			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain]\n");
-			if (listKernel) printf("  [-odd-order band|group|plain]\n");
+			else printf("  [-odd-pairs auto|on|off]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			printf("  [-launch-events on|off]\n");
 			return fail("kernel help requested.");

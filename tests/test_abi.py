@@ -150,11 +150,18 @@ def test_aa_rejects_list_only_argument(lbm, args):
         lbm.Kernel("aa-soa", geo, args)
 
 
+def test_list_rejects_aa_only_argument(lbm):
+    geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
+    with pytest.raises(RuntimeError, match="unknown kernel parameter"):
+        lbm.Kernel("list-aa-pv-soa", geo, ["-odd-pairs", "on"])
+
+
 @pytest.mark.parametrize("args,msg", [
     (["-prec", "fp8"], "fp64, fp32s or fp16s"),
     (["-fp16-scale", "3000"], "power of two"),
     (["-even-cap", "maybe"], "on, off or force"),
     (["-vec", "2"], "auto or 1"),
+    (["-odd-pairs", "sometimes"], "auto, on or off"),
 ])
 def test_aa_argument_errors(lbm, args, msg):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))

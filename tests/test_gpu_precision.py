@@ -272,7 +272,7 @@ def test_aa_shifted_bitexact(lbm, oracle, case, prec):
     base = ["-prec", prec, "-lambda", repr(LAM)]
     want = oracle.aa_run_shifted(mask, dims, per, [60, 40], omega, F, prec, magic=LAM)
     for extra in ([], ["-vec", "1"], ["-nt", "on"], ["-even-cap", "force"], ["-even-cap", "force", "-vec", "1"],
-                  ["-even-cap", "off"]):
+                  ["-even-cap", "off"], ["-odd-pairs", "on"], ["-odd-pairs", "on", "-even-cap", "force"]):
         print(f"aa case {case} -prec {prec} {extra}", flush=True)
         got = _aa_gpu(lbm, geo, [60, 40], omega, F, base + extra)
         assert np.array_equal(got, want), (extra, np.abs(got - want).max())
@@ -291,7 +291,7 @@ def test_aa_shifted_long_rows_and_nodes(lbm, oracle, prec):
     nodes = np.tile(oracle.W, (geo.n_fluid, 1)) + 1e-3 * rng.standard_normal((geo.n_fluid, 19))
     print(f"aa porous 24x20x300 -prec {prec}", flush=True)
     want = oracle.aa_run_shifted(m, geo.dims, geo.periodic, [40, 20], 1.25, 3e-5, prec, magic=LAM, nodes=nodes)
-    for extra in ([], ["-vec", "1"]):
+    for extra in ([], ["-vec", "1"], ["-odd-pairs", "on"]):
         got = _aa_gpu(lbm, geo, [40, 20], 1.25, 3e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra, nodes=nodes)
         assert np.array_equal(got, want), extra
     # the list kernel gives the same bits here (no open faces)
@@ -314,7 +314,7 @@ def test_aa_shifted_open_faces_ragged_rows(lbm, oracle, prec):
         mask = np.ones(int(np.prod(dims)), dtype=np.int32)
         geo = lbm.Geometry.from_mask(mask, dims, per)
         want = oracle.aa_run_shifted(mask, dims, per, [20, 10], 1.4, 5e-5, prec, magic=LAM)
-        for extra in ([], ["-vec", "1"], ["-even-cap", "force"]):
+        for extra in ([], ["-vec", "1"], ["-even-cap", "force"], ["-odd-pairs", "on"]):
             print(f"aa open {dims} {per} -prec {prec} {extra}", flush=True)
             got = _aa_gpu(lbm, geo, [20, 10], 1.4, 5e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
             assert np.array_equal(got, want), (dims, extra)
@@ -330,7 +330,7 @@ def test_aa_shifted_rows_over_several_blocks(lbm, oracle, prec):
     mask = (rng.random(int(np.prod(dims))) > 0.05).astype(np.int32)
     geo = lbm.Geometry.from_mask(mask, dims, per)
     want = oracle.aa_run_shifted(mask, dims, per, [12, 8], 1.3, 2e-5, prec, magic=LAM)
-    for extra in ([], ["-vec", "1"]):
+    for extra in ([], ["-vec", "1"], ["-odd-pairs", "on"]):
         print(f"aa rows {dims} -prec {prec} {extra}", flush=True)
         got = _aa_gpu(lbm, geo, [12, 8], 1.3, 2e-5, ["-prec", prec, "-lambda", repr(LAM)] + extra)
         assert np.array_equal(got, want), extra
