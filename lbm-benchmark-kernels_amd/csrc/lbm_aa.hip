@@ -658,7 +658,7 @@ static bool aa_output_sync(AaData * A)
 // pairs), whose blocks take several rows, takes the power of two from
 // `least` to 256 that leaves the fewest idle threads in a row's last block
 // (384 cells: 3 x 128; 32 FP32 pairs of a 64-cell row: 32, two rows per
-// wavefront).  `least` keeps a row segment at 128 B or more.
+// wavefront).
 static int odd_row_threads(int n)
 {
 	const int parts = (n + 255) / 256;
@@ -729,8 +729,9 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	// kernel's even step (stream_shape)
 	const int vec = c.args.evenVec == 0 ? 1 : c.args.prec == PREC_FP32S ? 2 : c.args.prec == PREC_FP16S ? 4 : 1;
 	const int zq = (g.Z + vec - 1) / vec;
-	const int laneBytes = (int)elem_bytes(c.args.prec) * vec;
-	const int ebs = even_row_threads(zq, laneBytes >= 8 ? 16 : 128 / laneBytes);
+	// groups of cells (8 bytes per lane): row segments of 16 lanes still span
+	// a 128-B line; one value per lane keeps the former 64-lane minimum
+	const int ebs = even_row_threads(zq, vec > 1 ? 16 : 64);
 	const unsigned egx = (unsigned)((zq + ebs - 1) / ebs);
 	// Defaults measured at 64^3 and 256^3 (duct channel): on lattices larger
 	// than the 256 MiB last-level cache non-temporal accesses (FP64 with the
