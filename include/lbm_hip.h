@@ -23,9 +23,11 @@
  *                     list-aa-pv-soa -- aa-soa's blocks follow its z rows)
  *   -adj rle|full     list only: the odd step reads the adjacency run-compressed
  *                     (default) or in full (18 entries per node)
- *   -nt on|off        non-temporal loads/stores in the even step (default on
- *                     for list-aa-pv-soa; for aa-soa on for lattices over
- *                     256 MiB, off for cache-sized ones: measured faster)
+ *   -nt on|off        non-temporal loads/stores in the even step (default:
+ *                     on when what a step touches -- PDFs, and for the list
+ *                     kernel its adjacency -- exceeds 256 MiB, the device's
+ *                     last-level cache; off for cache-sized lattices, which
+ *                     stream faster with plain accesses)
  *   -even-cap on|off|force
  *                     even step in 512-thread blocks, one block per compute
  *                     unit (default on for lattices of at least 16 such

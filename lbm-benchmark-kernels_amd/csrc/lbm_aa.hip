@@ -738,7 +738,7 @@ static bool aa_run(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	// occupancy cap, the narrow types at full occupancy), on smaller ones plain
 	// accesses with the cap (non-temporal accesses would push the
 	// cache-resident lattice out).
-	const bool big = aa_bytes(A) > (size_t)256 << 20;
+	const bool big = aa_bytes(A) > kLastLevelCacheBytes;
 	const StreamShape es = (big && c.args.prec != PREC_FP64 && !c.args.evenCapGiven)
 	                       ? StreamShape{ebs, 0} : stream_shape(&c, ebs, (uint64_t)egx * ebs * g.Y * g.X);
 	// at least one wavefront per block when rows are shorter than one

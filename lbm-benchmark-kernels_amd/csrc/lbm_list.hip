@@ -608,12 +608,21 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const AdjFull full = {L->dAdj, (uint32_t)L->adjStride};
 	const AdjRuns runs = {L->dRunMask, L->dRunBase, L->dRunAdj, (uint32_t)L->runStride};
 	const int vec = list_even_vec(L);
+	// non-temporal even-step accesses by default only when what a step
+	// touches -- the fluid nodes' PDFs and the odd step's adjacency --
+	// exceeds the last-level cache (256^3 porous FP64: 0.69 against
+	// 0.73 ms); a cache-sized lattice streams faster with plain accesses
+	// (64^3 channel FP64 15.6 -> 18.4 GFLUP/s, 128^3 porous FP32 31.7 ->
+	// 38.8; 128^3 porous FP64, 290 MB, 20.2 with and 17.4 without)
+	const size_t touched = elem_bytes(c.args.prec) * 19 * (size_t)L->nFluid +
+	                       (c.args.adjRle ? 72 * (size_t)L->nRunsWave + 12 * (size_t)L->nWaves : 72 * (size_t)L->nFluid);
+	const bool nt = c.args.nontemporal == 1 || (c.args.nontemporal == -1 && touched > kLastLevelCacheBytes);
 	// four FP16 nodes per thread run faster at full occupancy (measured +6 % at
 	// 256^3 porous); the other storage types with the occupancy cap
 	const StreamShape es = (vec == 4 && !c.args.evenCapGiven) ? StreamShape{bs, 0}
 	                       : stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
 	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, order, c.stream, nF, nC,
-	                       c.args.adjRle != 0, c.args.nontemporal != 0};   // default on (-1)
+	                       c.args.adjRle != 0, nt};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto even = [&](bool fl) {
 		switch (c.args.prec) {

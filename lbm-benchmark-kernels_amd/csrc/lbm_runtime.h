@@ -60,6 +60,12 @@ struct KernelArgs {
 // PDF storage of the list kernel.
 enum Prec { PREC_FP64 = 0, PREC_FP32S = 1, PREC_FP16S = 2 };
 
+// MI355X's last-level (memory-side) cache.  A lattice whose PDFs fit stays
+// cached from step to step, and the streaming even step then runs faster with
+// plain than with non-temporal accesses (which would push it out); above this
+// size non-temporal accesses are faster.  Both kernels' -nt default.
+constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
+
 // IEEE binary16 conversions on the host, round to nearest even (the
 // rounding of the device's v_cvt_f16_f32).
 uint16_t f32_to_f16(float f);
