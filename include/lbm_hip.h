@@ -266,9 +266,10 @@ int LbmHipBatchGetTuning(LbmHipBatch * b, LbmHipBatchTuning * out);
 int LbmHipBatchSetTuning(LbmHipBatch * b, const LbmHipBatchTuning * t);
 /* Times `pairs` launches of every candidate shape on the active domains and
  * keeps the fastest, then `pairs` step pairs for groups of 0 (none), 64,
- * 128, 192, 256, 384, 512, 768 and 1024 MiB with that shape, then the
- * shapes again under the chosen groups; the lattices are restored
- * afterwards.  Odd-step shapes include oddGroup 8, 32 and 128. */
+ * 128, 192, 256, 384, 512, 768 and 1024 MiB with that shape, each with plain
+ * and with non-temporal even-step accesses, then the shapes again under the
+ * chosen groups; the lattices are restored afterwards.  Odd-step shapes
+ * include oddGroup 8, 32 and 128. */
 int LbmHipBatchAutotune(LbmHipBatch * b, int pairs, LbmHipBatchTuning * out);
 /* Summed GPU time of all intervals and fluid-node updates done. */
 int LbmHipBatchStats(LbmHipBatch * b, double * gpuSeconds, double * flups);
