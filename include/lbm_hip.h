@@ -235,7 +235,11 @@ typedef struct {
 	                       interval is replayed from captured graphs, one
 	                       per group */
 	double pairMs;      /* per step pair with the chosen groups (autotune) */
-	int nontemporal;    /* 1: non-temporal loads/stores in the even step (-nt) */
+	int nontemporal;    /* even-step loads/stores: 1 non-temporal, 0 plain,
+	                       -1 (default, unless the batch was created with -nt
+	                       on|off) per group: non-temporal only when the
+	                       group's fluid PDFs and adjacency exceed the 256 MiB
+	                       last-level cache */
 	int oddGroup;       /* odd step: consecutive chunks each XCD takes per turn
 	                       (a power of two up to 256; 0: 8) -- neighbouring
 	                       chunks of one domain then share an L2 */
