@@ -62,6 +62,10 @@
  *                     device's 8 XCDs -- band (default): each XCD takes the
  *                     blocks of one eighth of the y range; group: 8
  *                     consecutive blocks per XCD in turn; plain: launch order
+ *   -even-order up|down
+ *                     list only: even-step blocks in ascending (default) or
+ *                     descending node order (starting on the planes the odd
+ *                     step ended on)
  *   -graph on|off     capture the step launches of a Kernel() call as a HIP
  *                     graph and replay it while steps, omega and force stay
  *                     the same (default off; not with -launch-events, VTK

@@ -262,20 +262,33 @@ __global__ __launch_bounds__(256) void k_run_compact(const uint32_t * __restrict
 // ---------------------------------------------------------------- steps
 // The per-node bodies (even_node, even_nodes, odd_node) live in lbm_list.h,
 // shared with the multi-lattice batch kernels.
+
+// Node block of the even step: ascending (default), or (down) descending,
+// so that the step starts on the highest nodes -- the x planes every XCD's
+// y band of the preceding odd step ends on, whose lines may still be in the
+// last-level cache -- and ends on the lowest, where the next odd step
+// begins.  At 256^3 porous FP64 descending measured 4 % faster on one
+// device and 0.5 % slower on another (eight alternating runs each).
+__device__ __forceinline__ uint32_t even_block(uint32_t down)
+{
+	return down ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+}
+
 template <class Codec, bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_list_even(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
-                                                   TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+                                                   TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft, uint32_t down)
 {
-	const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t n = even_block(down) * blockDim.x + threadIdx.x;
 	if (n >= nFluid) return;
 	even_node<Codec, FORCE_LOAD, NT>(pdf, cod, n, nCells, k, ft);
 }
 
 template <class Codec, int V, bool FORCE_LOAD, bool NT>
 __global__ __launch_bounds__(512) void k_list_even_v(typename Codec::S * pdf, Codec cod, uint32_t nFluid, uint32_t nCells,
-                                                     TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft)
+                                                     TrtC<typename Codec::R> k, ForceT<typename Codec::R> ft,
+                                                     uint32_t down)
 {
-	const uint32_t n0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
+	const uint32_t n0 = (even_block(down) * blockDim.x + threadIdx.x) * V;
 	if (n0 >= nFluid) return;
 	even_nodes<Codec, V, FORCE_LOAD, NT>(pdf, cod, n0, nFluid, nCells, k, ft);
 }
@@ -490,6 +503,7 @@ struct LaunchCfg {
 	hipStream_t stream;
 	uint32_t nF, nC;
 	bool rle, nt;
+	uint32_t down;          // even step in descending block order
 	// blocks of the even step when each thread takes v nodes
 	unsigned evenThreads(int v) const
 	{
@@ -507,16 +521,16 @@ static void launch_even(const LaunchCfg & c, typename Codec::S * pdf, Codec cod,
 	const ForceT<R> ft = force_cast<R>(f0);
 	if constexpr (V == 1) {
 		const dim3 g(c.evenThreads(1));
-		if (fl && c.nt)  k_list_even<Codec, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else if (fl)     k_list_even<Codec, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else if (c.nt)   k_list_even<Codec, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else             k_list_even<Codec, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		if (fl && c.nt)  k_list_even<Codec, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else if (fl)     k_list_even<Codec, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else if (c.nt)   k_list_even<Codec, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else             k_list_even<Codec, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
 	} else {
 		const dim3 g(c.evenThreads(V));
-		if (fl && c.nt)  k_list_even_v<Codec, V, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else if (fl)     k_list_even_v<Codec, V, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else if (c.nt)   k_list_even_v<Codec, V, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
-		else             k_list_even_v<Codec, V, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft);
+		if (fl && c.nt)  k_list_even_v<Codec, V, true, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else if (fl)     k_list_even_v<Codec, V, true, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else if (c.nt)   k_list_even_v<Codec, V, false, true><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
+		else             k_list_even_v<Codec, V, false, false><<<g, c.eblock, c.elds, c.stream>>>(pdf, cod, c.nF, c.nC, k, ft, c.down);
 	}
 }
 
@@ -622,7 +636,7 @@ static bool list_steps(KernelData * kd, const CaseData * cd, LatticeDesc * ld)
 	const StreamShape es = (vec == 4 && !c.args.evenCapGiven) ? StreamShape{bs, 0}
 	                       : stream_shape(&c, bs, ((uint64_t)L->nFluid + vec - 1) / vec);
 	const LaunchCfg cfg = {dim3((unsigned)es.block), es.lds, grid, block, order, c.stream, nF, nC,
-	                       c.args.adjRle != 0, nt};
+	                       c.args.adjRle != 0, nt, c.args.evenDown ? 1u : 0u};
 	const float f16s = (float)(1 << c.args.f16Shift);
 	auto even = [&](bool fl) {
 		switch (c.args.prec) {

@@ -53,6 +53,7 @@ struct KernelArgs {
 	int lazyShadow = 0;    // list: host copy allocated at the first download, not at Init
 	int statistics = 0;    // list: KernelStatistics during the run (the reference's STATISTICS build)
 	int oddOrder = 2;      // list: odd-step block order, 0 plain, 1 XCD-grouped, 2 y bands per XCD
+	int evenDown = 0;      // list: even step in descending block order (starts where the odd step ended)
 	int launchEvents = 0;  // per-launch timing events (even/odd averages)
 	int graph = 0;         // replay the step loop of a Kernel() call from a captured HIP graph
 };

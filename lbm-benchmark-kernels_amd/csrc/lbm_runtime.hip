@@ -208,6 +208,12 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			else if (!strcmp(v, "group")) a->oddOrder = 1;
 			else if (!strcmp(v, "band")) a->oddOrder = 2;
 			else return fail("argument -odd-order requires band, group or plain.");
+		} else if (listKernel && (is("-even-order") || is("--even-order"))) {
+			if (i + 1 >= p->nKernelArgs) return fail("argument -even-order requires down or up.");
+			const char * v = p->KernelArgs[++i];
+			if (!strcmp(v, "down")) a->evenDown = 1;
+			else if (!strcmp(v, "up")) a->evenDown = 0;
+			else return fail("argument -even-order requires down or up.");
 		} else if (is("-statistics") || is("--statistics")) {
 			a->statistics = 1;
 		} else if (is("-launch-events") || is("--launch-events")) {
@@ -240,7 +246,7 @@ bool parse_kernel_args(const Parameters * p, KernelArgs * a, bool listKernel)
 			printf("  [-blk <n>] [-blk-[xyz] <n>]\n");
 			if (listKernel) printf("  [-pad auto|modulus_1+offset_1(,modulus_n+offset_n)*]\n");
 			printf("  [-prec fp64|fp32s|fp16s] [-fp16-scale <2^k>] [-vec auto|1] [-statistics] [-graph on|off]\n");
-			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain]\n");
+			if (listKernel) printf("  [-adj rle|full] [-shadow eager|lazy] [-odd-order band|group|plain] [-even-order down|up]\n");
 			else printf("  [-odd-pairs auto|on|off]\n");
 			printf("  [-force on|off] [-lambda <magic>] [-bs <threads per block>] [-nt on|off] [-even-cap on|off|force] [-quiet]\n");
 			printf("  [-launch-events on|off]\n");

@@ -134,6 +134,7 @@ def test_init_fails_loudly_without_device(lbm):
     (["-vec", "2"], "auto or 1"),
     (["-even-cap", "maybe"], "on, off or force"),
     (["-odd-order", "zigzag"], "band, group or plain"),
+    (["-even-order", "sideways"], "down or up"),
     (["-launch-events", "maybe"], "on or off"),
 ])
 def test_kernel_argument_errors(lbm, args, msg):
@@ -143,7 +144,7 @@ def test_kernel_argument_errors(lbm, args, msg):
 
 
 @pytest.mark.parametrize("args", [["-pad", "auto"], ["-adj", "full"], ["-shadow", "lazy"],
-                                  ["-odd-order", "band"]])
+                                  ["-odd-order", "band"], ["-even-order", "up"]])
 def test_aa_rejects_list_only_argument(lbm, args):
     geo = lbm.Geometry.from_string("box", (8, 8, 8), (1, 1, 0))
     with pytest.raises(RuntimeError, match="unknown kernel parameter"):

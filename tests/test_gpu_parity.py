@@ -340,7 +340,8 @@ def test_setnode_roundtrip_and_upload(lbm):
 @pytest.mark.parametrize("geom,dims", [("porous-3-0.75-9", (64, 48, 40)), ("blocks-3", (30, 24, 27))])
 def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
     # run-compressed and full adjacency, with and without non-temporal
-    # accesses, all give the oracle's result bit for bit
+    # accesses, in either even-step block order, all give the oracle's
+    # result bit for bit
     geo = lbm.Geometry.from_string(geom, dims)
     m = geo.mask.reshape(-1)
     want, _ = oracle.list_run(m, geo.dims, geo.periodic, 30, 1.25, 3e-5, vsize=1)
@@ -349,7 +350,8 @@ def test_adjacency_modes_and_streaming_hints(lbm, oracle, geom, dims):
                  ["-adj", "rle", "-even-cap", "off"], ["-adj", "full", "-even-cap", "off", "-bs", "512"],
                  ["-adj", "rle", "-even-cap", "force"], ["-adj", "full", "-even-cap", "force", "-nt", "off"],
                  ["-adj", "rle", "-odd-order", "group"], ["-adj", "full", "-odd-order", "plain"],
-                 ["-adj", "rle", "-odd-order", "band", "-bs", "64"]):
+                 ["-adj", "rle", "-odd-order", "band", "-bs", "64"], ["-adj", "rle", "-even-order", "down"],
+                 ["-adj", "full", "-even-order", "down", "-nt", "on", "-even-cap", "force"]):
         with lbm.Kernel("list-aa-pv-soa", geo, args) as k:
             info = k.list_info()
             assert info.adjRle == (args[1] == "rle")
