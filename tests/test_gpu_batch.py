@@ -131,6 +131,31 @@ def test_batch_membership_and_convergence(lbm, group_mb):
             assert st.converged == 1 and st.active == 0 and 0 <= st.relChange <= 1e-4 and st.steps < 40000
 
 
+def test_batch_with_an_empty_domain(lbm):
+    """An all-solid domain among others: it holds no node, converges at its
+    second check (no momentum to change) and leaves its neighbours' results
+    bit-identical to the lattices run alone; ungrouped and one group per
+    domain."""
+    args = ["-lambda", repr(LAM)]
+    geos = _geos(lbm)[:2]
+    empty = lbm.Geometry.from_mask(np.zeros(6 * 5 * 4, dtype=np.int32), (6, 5, 4))
+    want = [_alone(lbm, g, om, F, [40], args) for g, om, F in geos]
+    for group_mb in (0, 1):
+        with lbm.Batch(args) as b:
+            if group_mb:
+                b.set_tuning(512, 1, 1, 256, group_mb)
+            a = b.add(*geos[0])
+            e = b.add(empty, 1.0, 1e-5)
+            c = b.add(*geos[1])
+            assert b.step(20, 1e-6) == 3
+            assert b.step(20, 1e-6) == 2
+            st = b.status(e)
+            assert (st.nFluid, st.active, st.converged, st.steps) == (0, 0, 1, 40)
+            assert st.mass == 0.0 and st.momentumX == 0.0
+            for i, w in zip((a, c), want):
+                assert np.array_equal(b.dump(i), w), (group_mb, i)
+
+
 def test_batch_setnode_guard(lbm):
     g, om, F = _geos(lbm)[0]
     with lbm.Batch() as b:
